@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Throughput of the HIP renderer on the random-sphere workload, one GPU.
+
+A step is one full render of the workload: 1920x1080 pixels, 64 samples per
+pixel, max depth 16, over the seeded 22x22 random-sphere field (488 spheres;
+80 % Lambertian, 15 % metal, 5 % glass, 3 large spheres, ground sphere) --
+scene and paths are synthetic.  The scene is resident in HBM before the timed
+region (uploaded once, like the reference's scene initialisation that
+precedes its timed draw(), render.rs:137-150).
+
+Prints one JSON line with Msamples/s (one sample = one camera path carrying 4
+hero wavelengths), the roofline of the dominant kernel (BVH traversal:
+algorithmic node + primitive + ray-queue bytes per launch over its mean
+launch time, HIP events on the render stream) and the CPU baseline (the CPU
+oracle -- a C++ restatement of the reference's renderer -- timed on this
+host's cores over a bounded slice of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    "spheres": dict(scene="spheres", width=1920, height=1080, samples=64, max_depth=16),
+    "cornell": dict(scene="cornell", width=256, height=256, samples=16, max_depth=8),
+}
+HBM_PEAK_GBS = 8000.0
+RAY_STREAM_BYTES = 48  # per traced ray: queue index in/out (8), origin+direction (32), hit t+record (8)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, w, seconds):
+    from clovers_amd import abi
+    from oracle import py_oracle as po
+    lib = None
+    try:
+        lib = po.lib(po.build(native=True))
+        kind_note = "g++ -O3 -march=native"
+    except Exception as e:  # no compiler on the host: use the shipped portable build
+        log("native oracle build failed (%s); using the portable build" % e)
+        lib = po.lib()
+        kind_note = "g++ -O3 -march=x86-64-v3"
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    sc = po.OracleScene(scene, library=lib)
+    opts = abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1)
+    # calibrate on 2 rows, then time a slice of about `seconds`
+    t0 = time.perf_counter()
+    sc.render(opts, threads=threads, rows=(0, 2))
+    per_row = max((time.perf_counter() - t0) / 2, 1e-4)
+    rows = int(max(2, min(w["height"], seconds / per_row)))
+    r0 = (w["height"] - rows) // 2
+    t0 = time.perf_counter()
+    sc.render(opts, threads=threads, rows=(r0, r0 + rows))
+    dt = time.perf_counter() - t0
+    samples = rows * w["width"] * w["samples"]
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": "%d central rows of %dx%d at %d spp, depth %d (%d samples, %.1f s); CPU oracle = C++ "
+                      "restatement of the reference renderer (%s, %d threads over rows)"
+                      % (rows, w["width"], w["height"], w["samples"], w["max_depth"], samples, dt, kind_note, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="spheres", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the work would shard by "
+              "sample range with one all-reduce of the pixel sums at the end)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import clovers_amd
+    from clovers_amd import abi, synth
+
+    w = WORKLOADS[args.workload]
+    scene = getattr(synth, w["scene"])()
+    log("workload", args.workload, w)
+    r = clovers_amd.Renderer(0)
+    t0 = time.perf_counter()
+    sid = r.upload(scene)
+    log("scene upload %.1f ms (host build %.1f ms)" % ((time.perf_counter() - t0) * 1e3, r.counters()["build_ms"]))
+    opts = abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1)
+    for i in range(args.warmup):
+        r.render(sid, opts)
+        log("warmup %d done" % i)
+    gpu_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        img, _ = r.render(sid, opts)
+        gpu_ms.append(r.counters()["render_ms"])
+        log("step %d: %.1f ms device" % (i, gpu_ms[-1]))
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    samples = w["width"] * w["height"] * w["samples"]
+    value = samples / (ms_per_step / 1e3) / 1e6
+    assert np.all(np.isfinite(img))
+
+    # dominant kernel: traversal.  Per-launch durations from HIP events ...
+    r.render(sid, abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1,
+                                  flags=abi.FLAG_TIME_KERNELS))
+    ct = r.counters()
+    avg_launch_ms = ct["extend_ms"] / ct["extend_launches"]
+    log("traversal: %d launches, %.2f ms total, %.4f ms mean" % (ct["extend_launches"], ct["extend_ms"], avg_launch_ms))
+    # ... and algorithmic bytes per launch from the counting traversal
+    r.render(sid, abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1,
+                                  flags=abi.FLAG_COUNT))
+    cc = r.counters()
+    alg_bytes = (cc["node_visits"] * cc["node_bytes"] + cc["prim_tests"] * cc["prim_bytes"]
+                 + cc["rays"] * RAY_STREAM_BYTES)
+    per_launch = alg_bytes / cc["extend_launches"]
+    achieved = per_launch / (avg_launch_ms * 1e-3) / 1e9
+    log("rays %d (%.2f per sample), nodes/ray %.1f, prims/ray %.1f"
+        % (cc["rays"], cc["rays"] / samples, cc["node_visits"] / cc["rays"], cc["prim_tests"] / cc["rays"]))
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_extend (4-wide BVH traversal + primitive tests)",
+                "bytes_per_launch": int(per_launch), "mean_launch_ms": round(avg_launch_ms, 4),
+                "per_ray": {"node_visits": round(cc["node_visits"] / cc["rays"], 3),
+                            "prim_tests": round(cc["prim_tests"] / cc["rays"], 3),
+                            "node_bytes": cc["node_bytes"], "prim_bytes": cc["prim_bytes"],
+                            "stream_bytes": RAY_STREAM_BYTES},
+                "rays_per_sample": round(cc["rays"] / samples, 4)}
+    r.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(scene, w, args.cpu_seconds)
+        log("cpu baseline %.3f Msamples/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    line = {
+        "metric": "Msamples/s at 1 GPU", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%s %dx%d %dspp depth%d" % (w["scene"], w["width"], w["height"], w["samples"],
+                                                             w["max_depth"]),
+                   "width": w["width"], "height": w["height"], "spp": w["samples"], "max_depth": w["max_depth"],
+                   "samples_per_step": samples, "device_ms_per_step": round(float(np.mean(gpu_ms)), 3),
+                   "parallelism": "1 GPU, wavefront"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

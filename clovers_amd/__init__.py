@@ -1,0 +1,165 @@
+"""clovers_amd -- MI355X wavefront spectral path tracer (HIP, gfx950).
+
+Python host front of libclovers_hip.so, the C ABI in include/clovers_hip.h.
+The render path is `draw()`, the drop-in for the reference's
+`draw_cpu::draw(&GlobalOptions, &RenderOptions, &Scene, Sampler) -> Vec<Xyz<E>>`
+(clovers-cli/src/draw_cpu.rs:26-98): it returns the same pixel buffer, linear
+XYZ with white point E, row 0 = top, index x + row*W, mean over samples.
+
+There is no CPU fallback: if the HIP library or a GPU is missing, every entry
+point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclovers_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "clovers_hip.h")
+
+_lib = None
+
+
+class ClvError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError("libclovers_hip.so is not built (run __graft_entry__.build() or "
+                          "make -C clovers_amd/csrc); there is no CPU fallback")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.clv_abi_version.restype = C.c_uint32
+    lib.clv_create.restype = C.c_int
+    lib.clv_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.clv_destroy.argtypes = [vp]
+    lib.clv_last_error.restype = C.c_char_p
+    lib.clv_last_error.argtypes = [vp]
+    lib.clv_scene_upload.restype = C.c_int
+    lib.clv_scene_upload.argtypes = [vp, vp, C.POINTER(C.c_uint32)]
+    lib.clv_scene_free.restype = C.c_int
+    lib.clv_scene_free.argtypes = [vp, C.c_uint32]
+    lib.clv_render.restype = C.c_int
+    lib.clv_render.argtypes = [vp, C.c_uint32, vp, vp, vp]
+    lib.clv_primary_hits.restype = C.c_int
+    lib.clv_primary_hits.argtypes = [vp, C.c_uint32, vp, vp, vp]
+    lib.clv_counters_get.restype = C.c_int
+    lib.clv_counters_get.argtypes = [vp, vp]
+    lib.clv_spectral_power.restype = C.c_float
+    lib.clv_spectral_power.argtypes = [C.POINTER(C.c_float), C.c_uint32]
+    lib.clv_wavelength_into_xyz.restype = C.c_int
+    lib.clv_wavelength_into_xyz.argtypes = [C.c_uint32, C.POINTER(C.c_float)]
+    lib.clv_scene_validate.restype = C.c_int
+    lib.clv_scene_validate.argtypes = [vp, vp]
+    if lib.clv_abi_version() != abi.ABI_VERSION:
+        raise ImportError("libclovers_hip.so ABI %d != %d" % (lib.clv_abi_version(), abi.ABI_VERSION))
+    _lib = lib
+    return lib
+
+
+def spectral_power(xyz, lam):
+    """clovers::spectrum::spectral_power (spectrum.rs:46-56), the library's host implementation."""
+    a = (C.c_float * 3)(*[float(v) for v in xyz])
+    return float(load_library().clv_spectral_power(a, int(lam)))
+
+
+def wavelength_into_xyz(lam):
+    """clovers::wavelength::wavelength_into_xyz (wavelength.rs:62-83) as the library tabulates it."""
+    out = (C.c_float * 3)()
+    st = load_library().clv_wavelength_into_xyz(int(lam), out)
+    if st != abi.OK:
+        raise ClvError(st, "wavelength out of range")
+    return np.array(out[:], np.float32)
+
+
+def validate(scene):
+    """clv_scene_validate: host build only (no device) -> dict of the scene's
+    shape; raises ClvError with the builder's message if the scene is invalid."""
+    desc, keep = scene.as_ctypes()
+    info = abi.SceneInfo()
+    st = load_library().clv_scene_validate(C.byref(desc), C.byref(info))
+    if st != abi.OK:
+        raise ClvError(st, info.error.decode())
+    return {k: getattr(info, k) for k, _ in abi.SceneInfo._fields_ if k != "error"}
+
+
+class Renderer:
+    """One context on one HIP device (not re-entrant, like clv_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.clv_create(device, C.byref(h))
+        if st != abi.OK:
+            raise ClvError(st, "clv_create(%d) failed (no HIP device?)" % device)
+        self.h = h
+
+    def _check(self, st):
+        if st != abi.OK:
+            raise ClvError(st, self.lib.clv_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, scene):
+        """scene: clovers_amd.scenefile.SceneDescription -> scene id."""
+        desc, keep = scene.as_ctypes()
+        sid = C.c_uint32()
+        self._check(self.lib.clv_scene_upload(self.h, C.byref(desc), C.byref(sid)))
+        return sid.value
+
+    def free(self, scene_id):
+        self._check(self.lib.clv_scene_free(self.h, scene_id))
+
+    def render(self, scene_id, opts, want_sq=False):
+        """clv_render -> (xyz[H, W, 3] f32, mean of squared samples or None)."""
+        W, H = opts.width, opts.height
+        out = np.zeros((H, W, 3), np.float32)
+        sq = np.zeros((H, W, 3), np.float32) if want_sq else None
+        self._check(self.lib.clv_render(self.h, scene_id, C.byref(opts), out.ctypes.data,
+                                        sq.ctypes.data if sq is not None else None))
+        return out, sq
+
+    def primary_hits(self, scene_id, opts):
+        W, H = opts.width, opts.height
+        ids = np.zeros((H, W), np.uint32)
+        ts = np.zeros((H, W), np.float32)
+        self._check(self.lib.clv_primary_hits(self.h, scene_id, C.byref(opts), ids.ctypes.data, ts.ctypes.data))
+        return ids, ts
+
+    def counters(self):
+        c = abi.Counters()
+        self._check(self.lib.clv_counters_get(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in abi.Counters._fields_}
+
+
+def draw(scene, width, height, samples, max_depth, seed=1, device=0, **kw):
+    """draw_cpu::draw for RenderMode::PathTracing: scene -> XYZ<E> pixel buffer [H, W, 3]."""
+    with Renderer(device) as r:
+        sid = r.upload(scene)
+        out, _ = r.render(sid, abi.render_opts(width, height, samples, max_depth, seed=seed, **kw))
+        return out

@@ -1,0 +1,91 @@
+"""ctypes / numpy mirror of include/clovers_hip.h (the C ABI of the renderer).
+
+Every struct here must match the header byte for byte; tests/test_abi.py checks
+the sizes against the compiled library.
+"""
+import ctypes as C
+
+import numpy as np
+
+ABI_VERSION = 1
+NO_XFORM = 0xFFFFFFFF
+MISS = 0xFFFFFFFF
+
+# status codes
+OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, EINTERNAL = range(6)
+STATUS_NAMES = {0: "CLV_OK", 1: "CLV_EINVAL", 2: "CLV_ENOMEM", 3: "CLV_EDEVICE",
+                4: "CLV_EUNSUPPORTED", 5: "CLV_EINTERNAL"}
+
+# object kinds (objects.rs:59-87)
+OBJ_SPHERE, OBJ_QUAD, OBJ_TRIANGLE, OBJ_BOXY, OBJ_TRIANGLE_QUV = range(5)
+# xform ops (objects/translate.rs, objects/rotate.rs)
+XF_TRANSLATE, XF_ROTATE_Y = 0, 1
+MAX_XFORM_OPS = 4
+# material kinds (materials.rs:141-156)
+MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DISPERSIVE, MAT_DIFFUSE_LIGHT = range(5)
+# texture kinds (textures.rs:22-109)
+TEX_SOLID, TEX_ILLUMINANT_D65, TEX_ILLUMINANT_D50 = range(3)
+# BVH algorithms (bvh.rs:32-40)
+BVH_LAM, BVH_SAH = 0, 1
+GUIDE_OFF, GUIDE_ON = 0, 1
+
+OBJECT_DTYPE = np.dtype([
+    ("kind", "<u4"), ("material", "<u4"), ("priority", "<u4"), ("xform", "<u4"),
+    ("p", "<f4", (9,)), ("_pad", "<u4", (3,)),
+])
+assert OBJECT_DTYPE.itemsize == 64
+
+XFORM_OP_DTYPE = np.dtype([("kind", "<u4"), ("v", "<f4", (3,))])
+XFORM_DTYPE = np.dtype([("n_ops", "<u4"), ("_pad", "<u4", (3,)), ("ops", XFORM_OP_DTYPE, (MAX_XFORM_OPS,))])
+assert XFORM_DTYPE.itemsize == 80
+
+MATERIAL_DTYPE = np.dtype([
+    ("kind", "<u4"), ("texture", "<u4"), ("color", "<f4", (3,)), ("intensity", "<f4"),
+    ("fuzz", "<f4"), ("refractive_index", "<f4"), ("cauchy_a", "<f4"), ("cauchy_b", "<f4"),
+    ("thin_film", "<u4"), ("film_index", "<f4"), ("film_thickness", "<f4"), ("_pad", "<u4"),
+])
+assert MATERIAL_DTYPE.itemsize == 56, MATERIAL_DTYPE.itemsize
+
+
+class CameraInit(C.Structure):
+    _fields_ = [("look_from", C.c_float * 3), ("look_at", C.c_float * 3), ("up", C.c_float * 3),
+                ("vertical_fov", C.c_float), ("aperture", C.c_float), ("focus_distance", C.c_float),
+                ("time_0", C.c_float), ("time_1", C.c_float)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32),
+                ("n_objects", C.c_uint32), ("objects", C.c_void_p),
+                ("n_materials", C.c_uint32), ("materials", C.c_void_p),
+                ("n_xforms", C.c_uint32), ("xforms", C.c_void_p),
+                ("camera", CameraInit),
+                ("background", C.c_float * 3),
+                ("mis_bvh_algorithm", C.c_uint32)]
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
+                ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("sample_offset", C.c_uint32),
+                ("guiding", C.c_uint32), ("fp64_accum", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("node_visits", C.c_uint64),
+                ("prim_tests", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),
+                ("build_ms", C.c_double), ("extend_launches", C.c_uint32), ("passes", C.c_uint32),
+                ("bvh_nodes", C.c_uint32), ("bvh_width", C.c_uint32), ("node_bytes", C.c_uint32),
+                ("prim_bytes", C.c_uint32), ("n_prims", C.c_uint32), ("bvh_depth", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_objects", C.c_uint32), ("n_prims", C.c_uint32), ("n_lights", C.c_uint32),
+                ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32), ("light_nodes", C.c_uint32),
+                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("error", C.c_char * 256)]
+
+
+FLAG_TIME_KERNELS = 1
+FLAG_COUNT = 2
+
+
+def render_opts(width, height, samples, max_depth, seed=1, sample_offset=0, guiding=0, fp64_accum=0, flags=0):
+    return RenderOpts(width, height, samples, max_depth, seed, sample_offset, guiding, fp64_accum, flags)

@@ -1,0 +1,114 @@
+/*
+ * clv_layout.h -- HBM data layout shared by the host builder and the HIP
+ * kernels.  Plain structs, no HIP types, so both g++ and hipcc include it.
+ *
+ * Scene data (resident in HBM for the scene's lifetime):
+ *   nodes      BVH4 nodes, 128 B each, 128-B aligned: child boxes as SoA
+ *              (lo/hi per axis x 4 children) + 4 child words.
+ *   prims      variable-size primitive records in float4 units:
+ *                sphere 2 x float4 (32 B), triangle 4 x float4 (64 B),
+ *                quad 5 x float4 (80 B); word .w of float4 #0 is the header
+ *              (type | (xform+1) << 2), word .w of float4 #1 the primitive
+ *              index into prim_info.
+ *   prim_info  per primitive: material, reported id (object*8 + face).
+ *   xforms     instance chains (RotateY/Translate), outermost op first.
+ *   materials  per material: kind, SPD table offsets, scalar parameters.
+ *   spd        n x 400 f32 spectral tables (index = lambda - 380).
+ *   lights     the multiple-importance-sampling tree over priority objects:
+ *              light records + a binary tree for sampling + leaf weights for
+ *              the pdf (each leaf weighs 2^-depth, bvh/hitable_trait.rs:89-127).
+ */
+#ifndef CLV_LAYOUT_H
+#define CLV_LAYOUT_H
+#include <stdint.h>
+
+#define CLV_LAMBDA_MIN 380
+#define CLV_SPECTRUM 400
+
+/* primitive types in record headers */
+#define PT_SPHERE 0u
+#define PT_QUAD 1u
+#define PT_TRI 2u
+
+/* record sizes in float4 units */
+#define PREC_SPHERE 2u
+#define PREC_TRI 4u
+#define PREC_QUAD 5u
+
+/* BVH4 child word: interior node index, or a leaf, or empty */
+#define CHILD_LEAF 0x80000000u
+#define CHILD_EMPTY 0xFFFFFFFFu
+/* leaf: bit31 | count-1 (3 bits, <= 8 prims) << 28 | first record offset (float4 units, 28 bits) */
+#define LEAF_MAX_PRIMS 8u
+#define LEAF_OFFSET_MASK 0x0FFFFFFFu
+
+typedef struct {
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  uint32_t child[4];
+  uint32_t pad[4];
+} Bvh4Node; /* 128 bytes */
+
+typedef struct {
+  uint32_t material;
+  uint32_t report_id;    /* object_index * 8 + face */
+} PrimInfo;
+
+/* one instance op, outermost first */
+#define XF_TRANSLATE 0u
+#define XF_ROTATE_Y 1u
+typedef struct {
+  uint32_t kind;
+  float a, b, c;         /* TRANSLATE: offset; ROTATE_Y: a = sin(theta), b = cos(theta) */
+} XformOp;
+typedef struct {
+  uint32_t n_ops;
+  uint32_t pad[3];
+  XformOp ops[4];
+} Xform;                 /* 80 bytes */
+
+/* material classes used to sort hits into shading queues */
+#define MC_MISS 0u
+#define MC_LIGHT 1u
+#define MC_LAMBERTIAN 2u
+#define MC_METAL 3u
+#define MC_GLASS 4u
+#define MC_COUNT 5u
+
+typedef struct {
+  uint32_t kind;         /* clv_material_kind */
+  uint32_t mclass;       /* MC_* */
+  int32_t albedo_spd;    /* offset of the reflectance SPD in spd[] (or -1) */
+  int32_t emit_spd;      /* offset of the emission SPD (or -1) */
+  float fuzz;            /* metal */
+  float ior;             /* dielectric refractive index */
+  float cauchy_a, cauchy_b;
+  uint32_t thin_film;
+  float film_ior, film_thickness;
+  uint32_t pad;
+} DevMaterial;           /* 48 bytes */
+
+/* light (MIS target) record: the reference's Quad/Triangle/Sphere params */
+typedef struct {
+  uint32_t type;         /* PT_* */
+  uint32_t pad0[3];
+  float q[4];            /* quad/tri: q, sphere: center */
+  float u[4];            /* quad/tri: u; sphere: u[0] = radius */
+  float v[4];
+  float n[4];            /* unit normal, n[3] = d */
+  float w[4];            /* w = n/(n.n), w[3] = area */
+} DevLight;              /* 96 bytes */
+
+/* light tree node for sampling: child word = node index | LT_LEAF | leaf index, or LT_EMPTY */
+#define LT_LEAF 0x80000000u
+#define LT_EMPTY 0xFFFFFFFFu
+typedef struct {
+  uint32_t child[2];
+} DevLightNode;
+
+typedef struct {
+  float llc[3], horizontal[3], vertical[3], origin[3], u[3], v[3];
+  float lens_radius;
+  float pad;
+} DevCamera;
+
+#endif

@@ -1,0 +1,857 @@
+/*
+ * host_build.cpp -- host-side scene preparation for the HIP renderer.
+ *
+ * Runs once per scene before any render, like the reference's
+ * SceneFile::initialize (clovers-cli/src/scenefile.rs:34-117), and is compiled
+ * with -ffp-contract=off so that every precomputed f32 value (quad normals and
+ * plane constants, rotation sines, camera frame) is rounded exactly as the
+ * reference's constructors round it.
+ *
+ *  - spectra: the reference precomputes a 400-entry SPD per solid colour
+ *    (spectrum.rs:26-34) with Meng et al. 2015's f64 upsampling
+ *    (spectrum_grid.rs:16-153); here every colour the device can look up
+ *    (albedo, dielectric colour, emission, background) becomes such a table,
+ *    so the device only does integer-wavelength table loads;
+ *  - flattening: every top-level object becomes primitive records (a Boxy
+ *    becomes its six quads, boxy.rs:60-73) carrying its RotateY/Translate
+ *    chain, with the reference's culling boxes (rotate.rs:40-93,
+ *    translate.rs:41-49, quad.rs:256-257, triangle.rs:70-88);
+ *  - scene BVH: binned SAH over the culling boxes, collapsed to a 4-wide BVH
+ *    with 128-B nodes (not the reference's binary tree: only the closest hit
+ *    matters, and that does not depend on the tree);
+ *  - light tree: the multiple-importance-sampling tree over priority objects
+ *    IS shape-sensitive (its pdf averages children, bvh/hitable_trait.rs:89-127),
+ *    so it is built with the reference's own SAH/LAM rules.
+ */
+#include "host_build.h"
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <numeric>
+
+#include "tables/spectral_tables.inc"
+
+namespace clv {
+namespace {
+
+const float PI_F = 3.14159265358979323846f;
+const float EPS_RECT = 0.0001f; /* EPSILON_RECT_THICKNESS, lib.rs:116 */
+
+struct F3 {
+  float x, y, z;
+};
+inline F3 f3(float x, float y, float z) { F3 r = {x, y, z}; return r; }
+inline F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline F3 operator-(F3 a) { return f3(-a.x, -a.y, -a.z); }
+inline F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+inline F3 operator/(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline float length(F3 a) { return sqrtf(dot(a, a)); }
+inline F3 unit(F3 a) { return a / length(a); }
+inline float comp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+/* ----------------------------------------------------------- spectra --- */
+/* Meng et al. 2015 tristimulus -> spectral power at one wavelength, f64, in
+ * the arithmetic order of spectrum_grid.rs:16-153.  0 outside the grid or in a
+ * cell without data points. */
+double meng_power(double lambda, double X, double Y, double Z) {
+  double norm = 1.0 / (X + Y + Z);
+  if (!(norm < DBL_MAX)) return 0.0;
+  double cx = X * norm, cy = Y * norm;
+  double u = spec_xy_to_uv[0] * cx + spec_xy_to_uv[1] * cy + spec_xy_to_uv[2];
+  double v = spec_xy_to_uv[3] * cx + spec_xy_to_uv[4] * cy + spec_xy_to_uv[5];
+  if (u < 0.0 || u >= (double)SPEC_GRID_W || v < 0.0 || v >= (double)SPEC_GRID_H) return 0.0;
+  const int iu = (int)u, iv = (int)v;
+  const signed short *cell = spec_cells[iu + SPEC_GRID_W * iv];
+  const int inside = cell[0], npts = cell[1];
+  const signed short *pt = cell + 2;
+  if (npts == 0) return 0.0;
+  const double sb = (lambda - SPEC_LAMBDA_MIN) / (SPEC_LAMBDA_MAX - SPEC_LAMBDA_MIN) * ((double)SPEC_NSAMP - 1.0);
+  const int s0 = (int)sb;
+  const int s1 = (s0 + 1 < SPEC_NSAMP) ? s0 + 1 : SPEC_NSAMP - 1;
+  const double sf = sb - (double)s0;
+  double p[6];
+  for (int i = 0; i < npts; i++) p[i] = spec_point_spectrum[pt[i]][s0] * (1.0 - sf) + spec_point_spectrum[pt[i]][s1] * sf;
+  double res = 0.0;
+  if (inside == 1) {
+    const double fu = u - (double)iu, fv = v - (double)iv;
+    /* corner layout 2 3 / 0 1 */
+    res = p[0] * (1.0 - fu) * (1.0 - fv) + p[2] * (1.0 - fu) * fv + p[3] * fu * fv + p[1] * fu * (1.0 - fv);
+  } else {
+    /* triangle fan around point 0 */
+    const double ox = spec_point_uv[pt[0]][0], oy = spec_point_uv[pt[0]][1];
+    const double ex = u - ox, ey = v - oy;
+    double ax = spec_point_uv[pt[1]][0] - ox, ay = spec_point_uv[pt[1]][1] - oy;
+    double bary_u = ax * ey - ex * ay;
+    for (int i = 0; i + 1 < npts; i++) {
+      const int nxt = (i == npts - 2) ? 1 : i + 2;
+      const double bx = spec_point_uv[pt[nxt]][0] - ox, by = spec_point_uv[pt[nxt]][1] - oy;
+      const double bary_v = ex * by - bx * ey;
+      const double area = ax * by - bx * ay;
+      const double bu = bary_u / area, bv = bary_v / area, bw = 1.0 - bu - bv;
+      if (bu < 0.0 || bv < 0.0 || bw < 0.0) {
+        bary_u = -bary_v;
+        ax = bx;
+        ay = by;
+        continue;
+      }
+      res = p[0] * bw + p[i + 1] * bv + p[nxt] * bu;
+      break;
+    }
+  }
+  return res / norm;
+}
+
+/* gaussian lobe of the CIE 1931 analytic fit (wavelength.rs:62-66) */
+float lobe(float x, float a, float mu, float s1, float s2) {
+  float t = (x - mu) / (x < mu ? s1 : s2);
+  return a * expf(-(t * t) / 2.0f);
+}
+
+/* ------------------------------------------------------------- boxes --- */
+struct Box {
+  float lo[3], hi[3];
+};
+Box box_from(F3 a, F3 b) {  /* AABB::new_from_coords */
+  Box r;
+  for (int i = 0; i < 3; i++) {
+    r.lo[i] = fminf(comp(a, i), comp(b, i));
+    r.hi[i] = fmaxf(comp(a, i), comp(b, i));
+  }
+  return r;
+}
+void box_pad(Box *b) {      /* AABB::pad, aabb.rs:99-119 */
+  for (int i = 0; i < 3; i++) {
+    if (!(b->hi[i] - b->lo[i] >= EPS_RECT)) {
+      float lo = b->lo[i] - EPS_RECT, hi = b->hi[i] + EPS_RECT;
+      b->lo[i] = fminf(lo, hi);
+      b->hi[i] = fmaxf(lo, hi);
+    }
+  }
+}
+Box box_union(const Box &a, const Box &b) {
+  Box r;
+  for (int i = 0; i < 3; i++) {
+    r.lo[i] = fminf(a.lo[i], b.lo[i]);
+    r.hi[i] = fmaxf(a.hi[i], b.hi[i]);
+  }
+  return r;
+}
+Box box_empty() {
+  Box r;
+  for (int i = 0; i < 3; i++) { r.lo[i] = FLT_MAX; r.hi[i] = -FLT_MAX; }
+  return r;
+}
+float box_area(const Box &b) {
+  float ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+  return 2.0f * (ex * ey + ey * ez + ex * ez);
+}
+/* RotateY's box of an inner box (rotate.rs:52-93) */
+Box box_rotate_y(const Box &b, float s, float c) {
+  F3 mn = f3(INFINITY, INFINITY, INFINITY), mx = f3(-INFINITY, -INFINITY, -INFINITY);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int k = 0; k < 2; k++) {
+        float fi = (float)i, fj = (float)j, fk = (float)k;
+        float x = fi * b.hi[0] + (1.0f - fi) * b.lo[0];
+        float y = fj * b.hi[1] + (1.0f - fj) * b.lo[1];
+        float z = fk * b.hi[2] + (1.0f - fk) * b.lo[2];
+        F3 t = f3(c * x + s * z, y, -s * x + c * z);
+        mn = f3(fminf(mn.x, t.x), fminf(mn.y, t.y), fminf(mn.z, t.z));
+        mx = f3(fmaxf(mx.x, t.x), fmaxf(mx.y, t.y), fmaxf(mx.z, t.z));
+      }
+  return box_from(mn, mx);
+}
+
+/* ------------------------------------------------------- primitives --- */
+struct FlatPrim {
+  uint32_t type;      /* PT_* */
+  uint32_t xform;     /* device xform index + 1, 0 = none */
+  uint32_t info;      /* index into prim_info */
+  float d[20];        /* record payload (see write_record) */
+  Box cull;           /* culling box in world space */
+};
+
+/* Quad::new (quad.rs:248-270) / Triangle::new (triangle.rs:61-101) plane constants */
+struct Plane {
+  F3 q, u, v, n, w;
+  float d, area;
+};
+Plane make_plane(F3 q, F3 u, F3 v, bool tri) {
+  Plane p;
+  p.q = q; p.u = u; p.v = v;
+  F3 nn = cross(u, v);
+  p.n = unit(nn);
+  p.d = -(dot(p.n, q));
+  p.w = nn / dot(nn, nn);
+  p.area = tri ? length(nn) / 2.0f : length(nn);
+  return p;
+}
+Box plane_box(const Plane &p, bool tri) {
+  Box b;
+  if (!tri) {
+    b = box_from(p.q, p.q + p.u + p.v);
+  } else {
+    F3 c1 = p.q, c2 = p.q + p.u, c3 = p.q + p.v;
+    Box r;
+    for (int i = 0; i < 3; i++) {
+      r.lo[i] = fminf(fminf(comp(c1, i), comp(c2, i)), comp(c3, i));
+      r.hi[i] = fmaxf(fmaxf(comp(c1, i), comp(c2, i)), comp(c3, i));
+    }
+    b = r;
+  }
+  box_pad(&b);
+  return b;
+}
+
+void write_record(std::vector<float> &out, const FlatPrim &p) {
+  uint32_t hdr = p.type | (p.xform << 2);
+  float fh, fi;
+  memcpy(&fh, &hdr, 4);
+  memcpy(&fi, &p.info, 4);
+  const float *d = p.d;
+  float z = 0.0f;
+  if (p.type == PT_SPHERE) {        /* [c, hdr] [r, 0, 0, info] */
+    float r[8] = {d[0], d[1], d[2], fh, d[3], z, z, fi};
+    out.insert(out.end(), r, r + 8);
+  } else if (p.type == PT_TRI) {    /* [a, hdr] [b, info] [c, 0] [n, 0] */
+    float r[16] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], z, d[9], d[10], d[11], z};
+    out.insert(out.end(), r, r + 16);
+  } else {                          /* [q, hdr] [u, info] [v, d] [n, area] [w, 0] */
+    float r[20] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], d[12],
+                   d[9], d[10], d[11], d[16], d[13], d[14], d[15], z};
+    out.insert(out.end(), r, r + 20);
+  }
+}
+
+void quad_payload(FlatPrim *fp, const Plane &p) {
+  float *d = fp->d;
+  d[0] = p.q.x; d[1] = p.q.y; d[2] = p.q.z;
+  d[3] = p.u.x; d[4] = p.u.y; d[5] = p.u.z;
+  d[6] = p.v.x; d[7] = p.v.y; d[8] = p.v.z;
+  d[9] = p.n.x; d[10] = p.n.y; d[11] = p.n.z;
+  d[12] = p.d;
+  d[13] = p.w.x; d[14] = p.w.y; d[15] = p.w.z;
+  d[16] = p.area;
+}
+
+/* ------------------------------------------------------ binned SAH BVH --- */
+struct BNode {
+  Box box;
+  int32_t left = -1, right = -1;  /* binary children (interior) */
+  uint32_t first = 0, count = 0;  /* leaf range in the permutation */
+};
+
+struct BvhBuilder {
+  const std::vector<FlatPrim> &prims;
+  std::vector<uint32_t> perm;
+  std::vector<F3> centroid;
+  std::vector<BNode> nodes;
+  uint32_t max_depth = 0;
+  static const int BINS = 16;
+  static const uint32_t MAX_LEAF = 4;
+
+  explicit BvhBuilder(const std::vector<FlatPrim> &p) : prims(p) {
+    perm.resize(p.size());
+    std::iota(perm.begin(), perm.end(), 0u);
+    centroid.resize(p.size());
+    for (size_t i = 0; i < p.size(); i++) {
+      const Box &b = p[i].cull;
+      centroid[i] = f3(0.5f * (b.lo[0] + b.hi[0]), 0.5f * (b.lo[1] + b.hi[1]), 0.5f * (b.lo[2] + b.hi[2]));
+    }
+    nodes.reserve(2 * p.size());
+  }
+
+  int build(uint32_t begin, uint32_t end, uint32_t depth) {
+    if (depth > max_depth) max_depth = depth;
+    int idx = (int)nodes.size();
+    nodes.emplace_back();
+    Box box = box_empty(), cbox = box_empty();
+    for (uint32_t i = begin; i < end; i++) {
+      box = box_union(box, prims[perm[i]].cull);
+      F3 c = centroid[perm[i]];
+      for (int a = 0; a < 3; a++) {
+        cbox.lo[a] = fminf(cbox.lo[a], comp(c, a));
+        cbox.hi[a] = fmaxf(cbox.hi[a], comp(c, a));
+      }
+    }
+    nodes[idx].box = box;
+    uint32_t n = end - begin;
+    if (n <= 2) { nodes[idx].first = begin; nodes[idx].count = n; return idx; }
+    /* binned SAH */
+    int best_axis = -1, best_split = -1;
+    float best_cost = FLT_MAX;
+    for (int a = 0; a < 3; a++) {
+      float ext = cbox.hi[a] - cbox.lo[a];
+      if (!(ext > 0.0f)) continue;
+      uint32_t cnt[BINS] = {0};
+      Box bb[BINS];
+      for (int k = 0; k < BINS; k++) bb[k] = box_empty();
+      float scale = (float)BINS / ext;
+      for (uint32_t i = begin; i < end; i++) {
+        int k = (int)((comp(centroid[perm[i]], a) - cbox.lo[a]) * scale);
+        k = k < 0 ? 0 : (k >= BINS ? BINS - 1 : k);
+        cnt[k]++;
+        bb[k] = box_union(bb[k], prims[perm[i]].cull);
+      }
+      float right_area[BINS];
+      uint32_t right_cnt[BINS];
+      Box acc = box_empty();
+      uint32_t c = 0;
+      for (int k = BINS - 1; k > 0; k--) {
+        acc = box_union(acc, bb[k]);
+        c += cnt[k];
+        right_area[k] = c ? box_area(acc) : 0.0f;
+        right_cnt[k] = c;
+      }
+      acc = box_empty();
+      c = 0;
+      for (int k = 0; k < BINS - 1; k++) {
+        acc = box_union(acc, bb[k]);
+        c += cnt[k];
+        if (c == 0 || right_cnt[k + 1] == 0) continue;
+        float cost = box_area(acc) * (float)c + right_area[k + 1] * (float)right_cnt[k + 1];
+        if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = k; }
+      }
+    }
+    float leaf_cost = box_area(box) * (float)n;
+    float split_cost = 0.3f * box_area(box) + best_cost;  /* traversal step ~0.3 prim tests */
+    if (best_axis < 0 || (n <= MAX_LEAF && split_cost >= leaf_cost)) {
+      if (n <= MAX_LEAF || best_axis < 0) {
+        if (n <= LEAF_MAX_PRIMS) { nodes[idx].first = begin; nodes[idx].count = n; return idx; }
+        /* degenerate centroids: split the range in half */
+        uint32_t mid = begin + n / 2;
+        int l = build(begin, mid, depth + 1);
+        int r = build(mid, end, depth + 1);
+        nodes[idx].left = l;
+        nodes[idx].right = r;
+        return idx;
+      }
+    }
+    float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
+    float scale = (float)BINS / ext;
+    float lo = cbox.lo[best_axis];
+    int a = best_axis, s = best_split;
+    uint32_t *mid_ptr = std::partition(perm.data() + begin, perm.data() + end, [&](uint32_t p) {
+      int k = (int)((comp(centroid[p], a) - lo) * scale);
+      k = k < 0 ? 0 : (k >= BINS ? BINS - 1 : k);
+      return k <= s;
+    });
+    uint32_t mid = (uint32_t)(mid_ptr - perm.data());
+    if (mid == begin || mid == end) mid = begin + n / 2;
+    int l = build(begin, mid, depth + 1);
+    int r = build(mid, end, depth + 1);
+    nodes[idx].left = l;
+    nodes[idx].right = r;
+    return idx;
+  }
+};
+
+struct Collapser {
+  const BvhBuilder &b;
+  const std::vector<FlatPrim> &prims;
+  std::vector<Bvh4Node> out;
+  std::vector<float> records;
+  uint32_t depth = 0;
+  Collapser(const BvhBuilder &bb, const std::vector<FlatPrim> &p) : b(bb), prims(p) {}
+
+  uint32_t leaf_word(const BNode &n) {
+    uint32_t off = (uint32_t)(records.size() / 4);
+    for (uint32_t i = 0; i < n.count; i++) write_record(records, prims[b.perm[n.first + i]]);
+    return CHILD_LEAF | ((n.count - 1u) << 28) | off;
+  }
+  uint32_t collapse(int bi, uint32_t d) {
+    if (d > depth) depth = d;
+    uint32_t idx = (uint32_t)out.size();
+    out.emplace_back();
+    std::vector<int> ch;
+    const BNode &root = b.nodes[bi];
+    if (root.count > 0) {
+      ch.push_back(bi);
+    } else {
+      ch.push_back(root.left);
+      ch.push_back(root.right);
+    }
+    while (ch.size() < 4) {
+      int pick = -1;
+      float best = -1.0f;
+      for (size_t i = 0; i < ch.size(); i++) {
+        const BNode &c = b.nodes[ch[i]];
+        if (c.count == 0) {
+          float ar = box_area(c.box);
+          if (ar > best) { best = ar; pick = (int)i; }
+        }
+      }
+      if (pick < 0) break;
+      int n = ch[pick];
+      ch[pick] = b.nodes[n].left;
+      ch.insert(ch.begin() + pick + 1, b.nodes[n].right);
+    }
+    Bvh4Node node;
+    memset(&node, 0, sizeof(node));
+    for (int k = 0; k < 4; k++) {
+      if (k < (int)ch.size()) {
+        const Box &bx = b.nodes[ch[k]].box;
+        node.lo_x[k] = bx.lo[0]; node.hi_x[k] = bx.hi[0];
+        node.lo_y[k] = bx.lo[1]; node.hi_y[k] = bx.hi[1];
+        node.lo_z[k] = bx.lo[2]; node.hi_z[k] = bx.hi[2];
+      } else {
+        node.lo_x[k] = node.lo_y[k] = node.lo_z[k] = FLT_MAX;
+        node.hi_x[k] = node.hi_y[k] = node.hi_z[k] = -FLT_MAX;
+        node.child[k] = CHILD_EMPTY;
+      }
+    }
+    for (int k = 0; k < (int)ch.size(); k++) {
+      const BNode &c = b.nodes[ch[k]];
+      node.child[k] = c.count > 0 ? leaf_word(c) : collapse(ch[k], d + 1);
+    }
+    out[idx] = node;
+    return idx;
+  }
+};
+
+/* ----------------------------------------- light tree (reference rules) --- */
+struct LItem {
+  Box box;          /* the reference's AABB of the hitable */
+  F3 centroid;      /* HitableTrait::centroid */
+  uint32_t light;   /* index into lights */
+};
+
+struct LightTreeBuilder {
+  std::vector<DevLightNode> nodes;
+  int alg;
+
+  static bool bounding(const std::vector<LItem> &v, Box *out) {
+    *out = box_empty();
+    if (v.empty()) return false;
+    Box b = v[0].box;
+    for (size_t i = 1; i < v.size(); i++) b = box_union(b, v[i].box);
+    *out = b;
+    return true;
+  }
+  uint32_t node(uint32_t c0, uint32_t c1) {
+    DevLightNode n;
+    n.child[0] = c0;
+    n.child[1] = c1;
+    nodes.push_back(n);
+    return (uint32_t)nodes.size() - 1;
+  }
+  static uint32_t leaf(const LItem &it) { return LT_LEAF | it.light; }
+
+  /* surface_area_heuristic.rs:134-156, including the AABB::default() start */
+  static float evaluate_sah(const std::vector<LItem> &v, int axis, float pos) {
+    Box lb, rb;
+    for (int i = 0; i < 3; i++) { lb.lo[i] = lb.hi[i] = 0.0f; rb.lo[i] = rb.hi[i] = 0.0f; }
+    uint64_t lc = 0, rc = 0;
+    for (const LItem &it : v) {
+      if (comp(it.centroid, axis) <= pos) { lc++; lb = box_union(lb, it.box); }
+      else { rc++; rb = box_union(rb, it.box); }
+    }
+    return (float)lc * box_area(lb) + (float)rc * box_area(rb);
+  }
+  /* surface_area_heuristic.rs:18-132 */
+  uint32_t sah(std::vector<LItem> v) {
+    Box b;
+    bounding(v, &b);
+    size_t n = v.size();
+    if (n == 1) return node(leaf(v[0]), LT_EMPTY);
+    if (n == 2) return node(leaf(v[0]), leaf(v[1]));
+    int best_axis = 0;
+    float best_pos = 0.0f, best_cost = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      float bmin = INFINITY, bmax = -INFINITY;
+      for (const LItem &it : v) { bmin = fminf(bmin, comp(it.centroid, a)); bmax = fmaxf(bmax, comp(it.centroid, a)); }
+      if (bmin == bmax) continue;
+      float scale = (bmax - bmin) / 8.0f;
+      for (int i = 0; i < 8; i++) {
+        float cand = bmin + (float)i * scale;
+        float cost = evaluate_sah(v, a, cand);
+        if (cost < best_cost) { best_pos = cand; best_axis = a; best_cost = cost; }
+      }
+    }
+    std::vector<LItem> L, R;
+    for (const LItem &it : v) (comp(it.centroid, best_axis) <= best_pos ? L : R).push_back(it);
+    if (L.empty()) {
+      LItem h = R[0];
+      R.erase(R.begin());
+      uint32_t r = sah(R);
+      return node(leaf(h), r);
+    }
+    if (R.empty()) {
+      LItem h = L[0];
+      L.erase(L.begin());
+      uint32_t l = sah(L);
+      return node(l, leaf(h));
+    }
+    uint32_t l = sah(L);
+    uint32_t r = sah(R);
+    return node(l, r);
+  }
+  /* longest_axis_midpoint.rs:12-87 (stable sort, strict less) */
+  uint32_t lam(std::vector<LItem> v) {
+    Box b;
+    bounding(v, &b);
+    float sp[3] = {b.hi[0] - b.lo[0], b.hi[1] - b.lo[1], b.hi[2] - b.lo[2]};
+    float largest = fmaxf(fmaxf(sp[0], sp[1]), sp[2]);
+    int axis = sp[0] == largest ? 0 : (sp[1] == largest ? 1 : 2);
+    auto less = [axis](const LItem &a, const LItem &c) { return a.box.lo[axis] < c.box.lo[axis]; };
+    size_t n = v.size();
+    if (n == 1) return node(leaf(v[0]), LT_EMPTY);
+    if (n == 2) return less(v[0], v[1]) ? node(leaf(v[0]), leaf(v[1])) : node(leaf(v[1]), leaf(v[0]));
+    std::stable_sort(v.begin(), v.end(), less);
+    if (n == 3) {
+      uint32_t r = node(leaf(v[1]), leaf(v[2]));
+      return node(leaf(v[0]), r);
+    }
+    size_t mid = n / 2;
+    std::vector<LItem> R(v.begin() + (long)mid, v.end());
+    v.resize(mid);
+    uint32_t l = lam(v);
+    uint32_t r = lam(R);
+    return node(l, r);
+  }
+};
+
+void leaf_weights(const std::vector<DevLightNode> &nodes, uint32_t word, float w, std::vector<float> &out) {
+  if (word == LT_EMPTY) return;
+  if (word & LT_LEAF) { out[word & ~LT_LEAF] += w; return; }
+  const DevLightNode &n = nodes[word];
+  if (n.child[1] == LT_EMPTY) { leaf_weights(nodes, n.child[0], w, out); return; }
+  if (n.child[0] == LT_EMPTY) { leaf_weights(nodes, n.child[1], w, out); return; }
+  leaf_weights(nodes, n.child[0], 0.5f * w, out);
+  leaf_weights(nodes, n.child[1], 0.5f * w, out);
+}
+
+bool finite_all(const float *p, int n) {
+  for (int i = 0; i < n; i++)
+    if (!isfinite(p[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+float spectral_power(const float xyz[3], int lambda) {
+  double p = meng_power((double)lambda, (double)xyz[0], (double)xyz[1], (double)xyz[2]) / spec_equal_energy_reflectance;
+  return (float)p;
+}
+
+void cie_xyz_table(float *out) {
+  for (int i = 0; i < CLV_SPECTRUM; i++) {
+    float l = (float)(CLV_LAMBDA_MIN + i);
+    float x = 0.0f + lobe(l, 1.056f, 599.8f, 37.9f, 31.0f) + lobe(l, 0.362f, 442.0f, 16.0f, 26.7f) +
+              lobe(l, -0.065f, 501.1f, 20.4f, 26.2f);
+    float y = lobe(l, 0.821f, 568.8f, 46.9f, 40.5f) + lobe(l, 0.286f, 530.9f, 16.3f, 31.1f);
+    float z = lobe(l, 1.217f, 437.0f, 11.8f, 36.0f) + lobe(l, 0.681f, 459.0f, 26.0f, 13.8f);
+    out[4 * i + 0] = x;
+    out[4 * i + 1] = y;
+    out[4 * i + 2] = z;
+    out[4 * i + 3] = 0.0f;
+  }
+}
+
+DevCamera make_camera(const clv_camera_init &ci, float aspect) {
+  DevCamera c;
+  memset(&c, 0, sizeof(c));
+  F3 from = f3(ci.look_from[0], ci.look_from[1], ci.look_from[2]);
+  F3 at = f3(ci.look_at[0], ci.look_at[1], ci.look_at[2]);
+  F3 up = f3(ci.up[0], ci.up[1], ci.up[2]);
+  float lens_radius = ci.aperture / 2.0f;
+  float theta = ci.vertical_fov * PI_F / 180.0f;
+  float half_height = tanf(theta / 2.0f);
+  float half_width = aspect * half_height;
+  F3 w = unit(from - at);
+  F3 u = unit(cross(up, w));
+  F3 v = unit(cross(w, u));
+  float fd = ci.focus_distance;
+  F3 llc = from - half_width * fd * u - half_height * fd * v - fd * w;
+  F3 hor = 2.0f * half_width * fd * u;
+  F3 ver = 2.0f * half_height * fd * v;
+  F3 *dst[6] = {(F3 *)c.llc, (F3 *)c.horizontal, (F3 *)c.vertical, (F3 *)c.origin, (F3 *)c.u, (F3 *)c.v};
+  F3 src[6] = {llc, hor, ver, from, u, v};
+  for (int i = 0; i < 6; i++) *dst[i] = src[i];
+  c.lens_radius = lens_radius;
+  return c;
+}
+
+int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!d || d->abi_version != CLV_ABI_VERSION) { *err = "descriptor missing or ABI version mismatch"; return CLV_EINVAL; }
+  if (d->n_objects == 0 || !d->objects) { *err = "scene has no objects"; return CLV_EINVAL; }
+  if (d->n_materials == 0 || !d->materials) { *err = "scene has no materials"; return CLV_EINVAL; }
+  if (d->n_xforms > 0 && !d->xforms) { *err = "xforms pointer is NULL"; return CLV_EINVAL; }
+  if (d->mis_bvh_algorithm > CLV_BVH_SAH) { *err = "unknown BVH algorithm"; return CLV_EINVAL; }
+  hs->camera = d->camera;
+  hs->n_objects = d->n_objects;
+
+  /* spectra */
+  auto add_table = [&](const float *vals) {
+    int32_t off = (int32_t)hs->spd.size();
+    hs->spd.insert(hs->spd.end(), vals, vals + CLV_SPECTRUM);
+    return off;
+  };
+  auto spd_of = [&](const float xyz[3]) {
+    float t[CLV_SPECTRUM];
+    for (int i = 0; i < CLV_SPECTRUM; i++) t[i] = spectral_power(xyz, CLV_LAMBDA_MIN + i);
+    return add_table(t);
+  };
+  auto illum_of = [&](uint32_t tex, float intensity, bool clamp01) {
+    const float *tab = tex == CLV_TEX_ILLUMINANT_D65 ? cie_d65 : cie_d50;
+    int first = tex == CLV_TEX_ILLUMINANT_D65 ? CIE_D65_FIRST_NM : CIE_D50_FIRST_NM;
+    int count = tex == CLV_TEX_ILLUMINANT_D65 ? CIE_D65_COUNT : CIE_D50_COUNT;
+    float t[CLV_SPECTRUM];
+    for (int i = 0; i < CLV_SPECTRUM; i++) {
+      int k = CLV_LAMBDA_MIN + i - first;
+      float p = (k >= 0 && k < count) ? tab[k] * intensity : 0.0f;
+      t[i] = clamp01 ? fminf(fmaxf(p, 0.0f), 1.0f) : p;
+    }
+    return add_table(t);
+  };
+  if (!finite_all(d->background, 3)) { *err = "background is not finite"; return CLV_EINVAL; }
+  hs->background_spd = spd_of(d->background);
+  for (uint32_t i = 0; i < d->n_materials; i++) {
+    const clv_material &m = d->materials[i];
+    DevMaterial dm;
+    memset(&dm, 0, sizeof(dm));
+    dm.kind = m.kind;
+    dm.albedo_spd = -1;
+    dm.emit_spd = -1;
+    dm.fuzz = m.fuzz;
+    dm.ior = m.refractive_index;
+    dm.cauchy_a = m.cauchy_a;
+    dm.cauchy_b = m.cauchy_b;
+    dm.thin_film = m.thin_film ? 1u : 0u;
+    dm.film_ior = m.film_index;
+    dm.film_thickness = m.film_thickness;
+    if (m.texture > CLV_TEX_ILLUMINANT_D50) { *err = "unknown texture kind"; return CLV_EINVAL; }
+    if (!finite_all(m.color, 3)) { *err = "material colour is not finite"; return CLV_EINVAL; }
+    switch (m.kind) {
+      case CLV_MAT_LAMBERTIAN:
+      case CLV_MAT_METAL:
+        /* albedo.color(): SolidColor -> SPD, illuminant -> clamp(table*intensity) */
+        dm.albedo_spd = m.texture == CLV_TEX_SOLID ? spd_of(m.color) : illum_of(m.texture, m.intensity, true);
+        dm.mclass = m.kind == CLV_MAT_LAMBERTIAN ? MC_LAMBERTIAN : MC_METAL;
+        break;
+      case CLV_MAT_DIELECTRIC:
+        dm.albedo_spd = spd_of(m.color); /* spectral_power(color, lambda), dielectric.rs:79-81 */
+        dm.mclass = MC_GLASS;
+        break;
+      case CLV_MAT_DISPERSIVE:
+        dm.mclass = MC_GLASS;
+        break;
+      case CLV_MAT_DIFFUSE_LIGHT:
+        /* emit: SolidColor -> color() (unclamped SPD), illuminant -> emit() (diffuse_light.rs:41-53) */
+        dm.emit_spd = m.texture == CLV_TEX_SOLID ? spd_of(m.color) : illum_of(m.texture, m.intensity, false);
+        dm.mclass = MC_LIGHT;
+        break;
+      default:
+        *err = "unsupported material kind";
+        return CLV_EUNSUPPORTED;
+    }
+    hs->materials.push_back(dm);
+  }
+
+  /* xforms: device copies with precomputed sin/cos (rotate.rs:40-45) */
+  for (uint32_t i = 0; i < d->n_xforms; i++) {
+    const clv_xform &x = d->xforms[i];
+    if (x.n_ops > CLV_MAX_XFORM_OPS) { *err = "too many xform ops"; return CLV_EINVAL; }
+    Xform dx;
+    memset(&dx, 0, sizeof(dx));
+    dx.n_ops = x.n_ops;
+    for (uint32_t k = 0; k < x.n_ops; k++) {
+      const clv_xform_op &op = x.ops[k];
+      if (!finite_all(op.v, 3)) { *err = "xform parameters are not finite"; return CLV_EINVAL; }
+      dx.ops[k].kind = op.kind;
+      if (op.kind == CLV_XF_TRANSLATE) {
+        dx.ops[k].a = op.v[0]; dx.ops[k].b = op.v[1]; dx.ops[k].c = op.v[2];
+      } else if (op.kind == CLV_XF_ROTATE_Y) {
+        float radians = op.v[0] * (PI_F / 180.0f);  /* f32::to_radians */
+        dx.ops[k].a = sinf(radians);
+        dx.ops[k].b = cosf(radians);
+      } else {
+        *err = "unknown xform op";
+        return CLV_EINVAL;
+      }
+    }
+    hs->xforms.push_back(dx);
+  }
+
+  /* flatten objects */
+  std::vector<FlatPrim> prims;
+  prims.reserve(d->n_objects);
+  std::vector<LItem> litems;
+  for (uint32_t i = 0; i < d->n_objects; i++) {
+    const clv_object &o = d->objects[i];
+    if (o.material >= d->n_materials) { *err = "object material index out of range"; return CLV_EINVAL; }
+    if (o.xform != CLV_NO_XFORM && o.xform >= d->n_xforms) { *err = "object xform index out of range"; return CLV_EINVAL; }
+    if (!finite_all(o.p, 9)) { *err = "object parameters are not finite"; return CLV_EINVAL; }
+    const float *p = o.p;
+    uint32_t xf = o.xform == CLV_NO_XFORM ? 0u : o.xform + 1u;
+    std::vector<FlatPrim> local;
+    Box obox;
+    switch (o.kind) {
+      case CLV_OBJ_SPHERE: {
+        FlatPrim fp;
+        memset(&fp, 0, sizeof(fp));
+        fp.type = PT_SPHERE;
+        fp.d[0] = p[0]; fp.d[1] = p[1]; fp.d[2] = p[2]; fp.d[3] = p[3];
+        F3 c = f3(p[0], p[1], p[2]);
+        float r = p[3];
+        obox = box_from(c - f3(r, r, r), c + f3(r, r, r));
+        local.push_back(fp);
+        if (o.priority && !xf) {
+          DevLight L;
+          memset(&L, 0, sizeof(L));
+          L.type = PT_SPHERE;
+          L.q[0] = p[0]; L.q[1] = p[1]; L.q[2] = p[2];
+          L.u[0] = r;
+          litems.push_back({obox, c, (uint32_t)hs->lights.size()});
+          hs->lights.push_back(L);
+        }
+        break;
+      }
+      case CLV_OBJ_QUAD:
+      case CLV_OBJ_TRIANGLE:
+      case CLV_OBJ_TRIANGLE_QUV: {
+        bool tri = o.kind != CLV_OBJ_QUAD;
+        F3 a = f3(p[0], p[1], p[2]), b = f3(p[3], p[4], p[5]), c = f3(p[6], p[7], p[8]);
+        Plane pl;
+        F3 va, vb, vc;  /* triangle corners for the watertight test */
+        if (o.kind == CLV_OBJ_TRIANGLE) {       /* Triangle::from_coordinates */
+          pl = make_plane(a, b - a, c - a, true);
+          va = a; vb = b; vc = c;
+        } else {                                /* Quad::new / Triangle::new(q, u, v) */
+          pl = make_plane(a, b, c, tri);
+          va = a; vb = a + b; vc = a + c;
+        }
+        obox = plane_box(pl, tri);
+        FlatPrim fp;
+        memset(&fp, 0, sizeof(fp));
+        if (tri) {
+          fp.type = PT_TRI;
+          float *q = fp.d;
+          q[0] = va.x; q[1] = va.y; q[2] = va.z;
+          q[3] = vb.x; q[4] = vb.y; q[5] = vb.z;
+          q[6] = vc.x; q[7] = vc.y; q[8] = vc.z;
+          q[9] = pl.n.x; q[10] = pl.n.y; q[11] = pl.n.z;
+        } else {
+          fp.type = PT_QUAD;
+          quad_payload(&fp, pl);
+        }
+        local.push_back(fp);
+        if (o.priority && !xf) {
+          DevLight L;
+          memset(&L, 0, sizeof(L));
+          L.type = tri ? PT_TRI : PT_QUAD;
+          L.q[0] = pl.q.x; L.q[1] = pl.q.y; L.q[2] = pl.q.z;
+          L.u[0] = pl.u.x; L.u[1] = pl.u.y; L.u[2] = pl.u.z;
+          L.v[0] = pl.v.x; L.v[1] = pl.v.y; L.v[2] = pl.v.z;
+          L.n[0] = pl.n.x; L.n[1] = pl.n.y; L.n[2] = pl.n.z; L.n[3] = pl.d;
+          L.w[0] = pl.w.x; L.w[1] = pl.w.y; L.w[2] = pl.w.z; L.w[3] = pl.area;
+          F3 cen = tri ? pl.q + (pl.u / 4.0f) + (pl.v / 4.0f) : pl.q + (pl.u / 2.0f) + (pl.v / 2.0f);
+          litems.push_back({obox, cen, (uint32_t)hs->lights.size()});
+          hs->lights.push_back(L);
+        }
+        break;
+      }
+      case CLV_OBJ_BOXY: {                      /* boxy.rs:43-83 */
+        F3 c0 = f3(p[0], p[1], p[2]), c1 = f3(p[3], p[4], p[5]);
+        F3 mn = f3(fminf(c0.x, c1.x), fminf(c0.y, c1.y), fminf(c0.z, c1.z));
+        F3 mx = f3(fmaxf(c0.x, c1.x), fmaxf(c0.y, c1.y), fmaxf(c0.z, c1.z));
+        F3 dx = f3(mx.x - mn.x, 0, 0), dy = f3(0, mx.y - mn.y, 0), dz = f3(0, 0, mx.z - mn.z);
+        F3 q[6] = {f3(mn.x, mn.y, mx.z), f3(mx.x, mn.y, mx.z), f3(mx.x, mn.y, mn.z),
+                   f3(mn.x, mn.y, mn.z), f3(mn.x, mx.y, mx.z), f3(mn.x, mn.y, mn.z)};
+        F3 uu[6] = {dx, -dz, -dx, dz, dx, dx};
+        F3 vv[6] = {dy, dy, dy, dy, -dz, dz};
+        for (int k = 0; k < 6; k++) {
+          FlatPrim fp;
+          memset(&fp, 0, sizeof(fp));
+          fp.type = PT_QUAD;
+          quad_payload(&fp, make_plane(q[k], uu[k], vv[k], false));
+          local.push_back(fp);
+        }
+        obox = box_from(c0, c1);
+        break;
+      }
+      default:
+        *err = "unknown object kind";
+        return CLV_EINVAL;
+    }
+    if (o.priority && (xf || o.kind == CLV_OBJ_BOXY)) {
+      *err = "a RotateY/Translate/Boxy object cannot be a light-sampling target (the reference's random() is unimplemented)";
+      return CLV_EUNSUPPORTED;
+    }
+    /* world culling box through the wrapper chain, innermost first */
+    if (xf) {
+      const clv_xform &x = d->xforms[o.xform];
+      for (int k = (int)x.n_ops - 1; k >= 0; k--) {
+        const Xform &dx = hs->xforms[o.xform];
+        if (x.ops[k].kind == CLV_XF_ROTATE_Y) {
+          obox = box_rotate_y(obox, dx.ops[k].a, dx.ops[k].b);
+        } else {
+          for (int a = 0; a < 3; a++) {
+            float off = x.ops[k].v[a];
+            float lo = obox.lo[a] + off, hi = obox.hi[a] + off;
+            obox.lo[a] = fminf(lo, hi);
+            obox.hi[a] = fmaxf(lo, hi);
+          }
+        }
+      }
+    }
+    /* widen by a few ulps so that the device slab test never culls a hit the
+     * primitive test accepts */
+    for (int a = 0; a < 3; a++) {
+      float s = 4.0e-6f * fmaxf(fabsf(obox.lo[a]), fabsf(obox.hi[a])) + 1.0e-6f;
+      obox.lo[a] -= s;
+      obox.hi[a] += s;
+    }
+    for (size_t k = 0; k < local.size(); k++) {
+      FlatPrim &fp = local[k];
+      fp.xform = xf;
+      fp.info = (uint32_t)hs->prim_info.size();
+      fp.cull = obox;
+      PrimInfo pi;
+      pi.material = o.material;
+      pi.report_id = i * 8u + (uint32_t)k;
+      hs->prim_info.push_back(pi);
+      prims.push_back(fp);
+    }
+  }
+  if (litems.empty()) {
+    *err = "at least one priority object is required (the reference panics, bvh/build/surface_area_heuristic.rs:23)";
+    return CLV_EINVAL;
+  }
+  hs->n_prims = (uint32_t)prims.size();
+
+  /* scene BVH */
+  BvhBuilder bb(prims);
+  int root = bb.build(0, (uint32_t)prims.size(), 0);
+  Collapser col(bb, prims);
+  col.out.reserve(bb.nodes.size() / 2 + 1);
+  col.records.reserve(prims.size() * 16);
+  col.collapse(root, 0);
+  hs->nodes.swap(col.out);
+  hs->prims.swap(col.records);
+  hs->bvh_depth = col.depth;
+  /* traversal keeps at most 3 pending children per level in a 72-entry stack */
+  if (3u * (col.depth + 1u) > 72u) { *err = "scene BVH deeper than the traversal stack allows"; return CLV_EUNSUPPORTED; }
+  if (hs->prims.size() / 4 > LEAF_OFFSET_MASK) { *err = "scene too large for 28-bit record offsets"; return CLV_EUNSUPPORTED; }
+
+  /* light tree */
+  LightTreeBuilder lt;
+  lt.alg = (int)d->mis_bvh_algorithm;
+  uint32_t lroot = d->mis_bvh_algorithm == CLV_BVH_SAH ? lt.sah(litems) : lt.lam(litems);
+  hs->light_nodes.swap(lt.nodes);
+  hs->light_root = lroot;
+  hs->light_weights.assign(hs->lights.size(), 0.0f);
+  leaf_weights(hs->light_nodes, lroot, 1.0f, hs->light_weights);
+  hs->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return CLV_OK;
+}
+
+}  // namespace clv

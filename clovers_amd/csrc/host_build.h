@@ -1,0 +1,47 @@
+/* host_build.h -- host-side scene preparation (spectra, flattening, BVHs). */
+#ifndef CLV_HOST_BUILD_H
+#define CLV_HOST_BUILD_H
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/clovers_hip.h"
+#include "clv_layout.h"
+
+namespace clv {
+
+struct HostScene {
+  std::vector<Bvh4Node> nodes;        /* root = nodes[0] */
+  std::vector<float> prims;           /* float4 records, 4 floats per unit */
+  std::vector<PrimInfo> prim_info;
+  std::vector<Xform> xforms;
+  std::vector<DevMaterial> materials;
+  std::vector<float> spd;             /* n x 400 */
+  int32_t background_spd = -1;
+  std::vector<DevLight> lights;
+  std::vector<DevLightNode> light_nodes;
+  std::vector<float> light_weights;   /* per light leaf: 2^-depth */
+  uint32_t light_root = LT_EMPTY;     /* child word of the tree root */
+  clv_camera_init camera;
+  uint32_t n_prims = 0;
+  uint32_t bvh_depth = 0;
+  uint32_t n_objects = 0;
+  double build_ms = 0.0;
+};
+
+/* Build everything the device needs from the descriptor.  Returns a
+ * clv_status; *err holds the detail on failure. */
+int build_host_scene(const clv_scene_desc *desc, HostScene *out, std::string *err);
+
+/* Camera::new (camera.rs:59-99) for the given aspect ratio. */
+DevCamera make_camera(const clv_camera_init &ci, float aspect);
+
+/* wavelength_into_xyz (wavelength.rs:62-83) for lambda = 380..779: 400 x float4 (x, y, z, 0). */
+void cie_xyz_table(float *out1600);
+
+/* spectral_power (spectrum.rs:46-56): exposed for the spectral tests. */
+float spectral_power(const float xyz[3], int lambda);
+
+}  // namespace clv
+#endif

@@ -1,0 +1,865 @@
+/*
+ * kernels.hip -- wavefront spectral path tracer for MI355X (gfx950).
+ *
+ * The reference evaluates one camera sample with a recursive function
+ * (clovers-cli/src/trace.rs:21-122) per pixel per row thread
+ * (draw_cpu.rs:101-146).  Here every sample is a path slot in HBM and a
+ * bounce is two kernels over all live paths:
+ *
+ *   extend  closest hit of every queued ray (4-wide BVH, short per-lane stack
+ *           in LDS); each hit is appended to one of five shading queues by
+ *           material class (miss / emitter / Lambertian / metal / glass) with
+ *           one wave-aggregated atomic per class -- ballot + popcount prefix.
+ *   shade   one launch over the concatenated class queues in 64-aligned
+ *           chunks, so every wave runs a single material's code; surviving
+ *           paths are appended to the next ray queue the same way.
+ *
+ * The iterative form accumulates L += T * emitted at each vertex and
+ * multiplies T by the vertex weight, which is the reference's recursion
+ * unrolled: specular vertices contribute atten * L_next (trace.rs:74-79), diffuse
+ * vertices emitted + atten * pdf_bsdf / pdf_mix * L_next (trace.rs:116-119).
+ * `resolve` applies the per-sample estimator of draw_cpu.rs:130-142 and sums
+ * samples per pixel in sample order.
+ */
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "clv_layout.h"
+#include "kernels.h"
+
+namespace clv {
+
+#define PI_F 3.14159265358979323846f
+#define EPS_ACNE 0.001f   /* EPSILON_SHADOW_ACNE, lib.rs:114 */
+#define EPS_RECT 0.0001f  /* EPSILON_RECT_THICKNESS, lib.rs:116 */
+#define MISS_WORD 0xFFFFFFFFu
+
+/* ------------------------------------------------------------- helpers --- */
+struct V {
+  float x, y, z;
+};
+__device__ __forceinline__ V mk(float x, float y, float z) { V r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ V operator+(V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V operator-(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V operator-(V a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V operator*(float s, V a) { return mk(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ V operator/(V a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+__device__ __forceinline__ V unit(V a) { return a / sqrtf(dot(a, a)); }
+__device__ __forceinline__ V xyz(float4 f) { return mk(f.x, f.y, f.z); }
+__device__ __forceinline__ float comp(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+__device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
+
+/* PCG32 (O'Neill 2014), bit-identical to the oracle's per-path streams */
+__device__ __forceinline__ uint32_t pcg_next(uint64_t &s) {
+  uint64_t old = s;
+  s = old * 6364136223846793005ull + 1442695040888963407ull;
+  uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+  uint32_t rot = (uint32_t)(old >> 59u);
+  return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ uint64_t pcg_for_path(uint64_t seed, uint32_t pixel, uint32_t sample) {
+  uint64_t s = splitmix64(seed ^ splitmix64(((uint64_t)sample << 32) | (uint64_t)pixel));
+  (void)pcg_next(s);
+  return s;
+}
+__device__ __forceinline__ float rnd(uint64_t &s) { return (float)(pcg_next(s) >> 8) * 0x1p-24f; }
+__device__ __forceinline__ bool coin(uint64_t &s) { return (pcg_next(s) >> 31) != 0u; }
+__device__ __forceinline__ float rnd_pm1(uint64_t &s) { return 2.0f * rnd(s) - 1.0f; }
+
+/* wave-aggregated append of `item` to queue q (counter *cnt) when pred */
+__device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *q, uint32_t *cnt) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  uint32_t lane = __lane_id();
+  uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+  base = __shfl(base, (int)leader);
+  uint64_t below = mask & ((1ull << lane) - 1ull);
+  if (pred) q[base + (uint32_t)__popcll(below)] = item;
+}
+
+/* -------------------------------------------------- primitive tests --- */
+/* Sphere::hit (objects/sphere.rs:70-125): strict t bounds, near root first */
+__device__ __forceinline__ bool sphere_t(V c, float r, V o, V d, float tmin, float tmax, float &t) {
+  V oc = o - c;
+  float a = dot(d, d);
+  float half_b = dot(oc, d);
+  float cc = dot(oc, oc) - r * r;
+  float disc = half_b * half_b - a * cc;
+  if (!(disc > 0.0f)) return false;
+  float root = sqrtf(disc);
+  float t0 = (-half_b - root) / a;
+  if (t0 < tmax && t0 > tmin) { t = t0; return true; }
+  float t1 = (-half_b + root) / a;
+  if (t1 < tmax && t1 > tmin) { t = t1; return true; }
+  return false;
+}
+
+/* Quad::hit (objects/quad.rs:88-132): |n.d| < 1e-4 rejected, inclusive t and (a, b) bounds */
+__device__ __forceinline__ bool quad_t(const float4 *rec, V o, V d, float tmin, float tmax, float &t) {
+  float4 f0 = rec[0], f1 = rec[1], f2 = rec[2], f3 = rec[3], f4 = rec[4];
+  V q = xyz(f0), u = xyz(f1), v = xyz(f2), n = xyz(f3), w = xyz(f4);
+  float denom = dot(n, d);
+  if (fabsf(denom) < EPS_RECT) return false;
+  float tt = (-f2.w - dot(n, o)) / denom;
+  if (tt < tmin || tt > tmax) return false;
+  V p = o + tt * d;
+  V ph = p - q;
+  float alpha = dot(w, cross(ph, v));
+  float beta = dot(w, cross(u, ph));
+  if (!(alpha >= 0.0f && alpha <= 1.0f && beta >= 0.0f && beta <= 1.0f)) return false;
+  t = tt;
+  return true;
+}
+
+/* Per-ray constants of the watertight triangle test (Woop, Benthin, Wald 2013) */
+struct WRay {
+  int kx, ky, kz;
+  float sx, sy, sz;
+};
+__device__ __forceinline__ WRay wray(V d) {
+  WRay w;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  w.kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+  w.kx = w.kz == 2 ? 0 : w.kz + 1;
+  w.ky = w.kx == 2 ? 0 : w.kx + 1;
+  if (comp(d, w.kz) < 0.0f) { int t = w.kx; w.kx = w.ky; w.ky = t; }
+  float dz = comp(d, w.kz);
+  w.sx = comp(d, w.kx) / dz;
+  w.sy = comp(d, w.ky) / dz;
+  w.sz = 1.0f / dz;
+  return w;
+}
+
+/* Watertight ray/triangle test with the reference's acceptance rules kept:
+ * |n.d| < 1e-4 is rejected and the t bounds are inclusive (triangle.rs:129-139). */
+__device__ __forceinline__ bool tri_t(const float4 *rec, V o, V d, const WRay &wr, float tmin, float tmax, float &t) {
+  float4 f0 = rec[0], f1 = rec[1], f2 = rec[2], f3 = rec[3];
+  V n = xyz(f3);
+  if (fabsf(dot(n, d)) < EPS_RECT) return false;
+  V A = xyz(f0) - o, B = xyz(f1) - o, C = xyz(f2) - o;
+  float Akx = comp(A, wr.kx), Aky = comp(A, wr.ky), Akz = comp(A, wr.kz);
+  float Bkx = comp(B, wr.kx), Bky = comp(B, wr.ky), Bkz = comp(B, wr.kz);
+  float Ckx = comp(C, wr.kx), Cky = comp(C, wr.ky), Ckz = comp(C, wr.kz);
+  float Ax = Akx - wr.sx * Akz, Ay = Aky - wr.sy * Akz;
+  float Bx = Bkx - wr.sx * Bkz, By = Bky - wr.sy * Bkz;
+  float Cx = Ckx - wr.sx * Ckz, Cy = Cky - wr.sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float Vv = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if ((U < 0.0f || Vv < 0.0f || W < 0.0f) && (U > 0.0f || Vv > 0.0f || W > 0.0f)) return false;
+  float det = U + Vv + W;
+  if (det == 0.0f) return false;
+  float T = U * (wr.sz * Akz) + Vv * (wr.sz * Bkz) + W * (wr.sz * Ckz);
+  float tt = T / det;
+  if (!(tt >= tmin && tt <= tmax)) return false;
+  t = tt;
+  return true;
+}
+
+/* RotateY/Translate chain, outermost op first (rotate.rs:97-113, translate.rs:60-66) */
+__device__ __forceinline__ void xform_ray(const Xform &x, V &o, V &d) {
+  for (uint32_t k = 0; k < x.n_ops; k++) {
+    const XformOp &op = x.ops[k];
+    if (op.kind == XF_TRANSLATE) {
+      o = o - mk(op.a, op.b, op.c);
+    } else {
+      float s = op.a, c = op.b;
+      o = mk(c * o.x - s * o.z, o.y, s * o.x + c * o.z);
+      d = unit(mk(c * d.x - s * d.z, d.y, s * d.x + c * d.z));
+    }
+  }
+}
+
+/* test one primitive record; on a hit inside [tmin, tbest] update tbest */
+__device__ __forceinline__ uint32_t prim_units(uint32_t type) {
+  return type == PT_SPHERE ? PREC_SPHERE : (type == PT_TRI ? PREC_TRI : PREC_QUAD);
+}
+
+template <bool HAS_XFORM>
+__device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec, uint32_t hdr, V o, V d, const WRay &wr,
+                                          float tmin, float tmax, float &t) {
+  uint32_t type = hdr & 3u;
+  uint32_t xf = hdr >> 2;
+  WRay w2 = wr;
+  if (HAS_XFORM && xf) {
+    xform_ray(sc.xforms[xf - 1u], o, d);
+    if (type == PT_TRI) w2 = wray(d);
+  }
+  if (type == PT_SPHERE) {
+    float4 f0 = rec[0], f1 = rec[1];
+    return sphere_t(xyz(f0), f1.x, o, d, tmin, tmax, t);
+  }
+  if (type == PT_TRI) return tri_t(rec, o, d, w2, tmin, tmax, t);
+  return quad_t(rec, o, d, tmin, tmax, t);
+}
+
+/* ------------------------------------------------------ traversal --- */
+#define STACK_LDS 24
+#define STACK_SPILL 48
+
+/* Closest hit over the 4-wide BVH.  Returns the record offset of the hit
+ * primitive (or MISS_WORD) and its distance in tbest.  Every primitive keeps
+ * its own acceptance rule relative to the current closest distance, as in the
+ * reference (inclusive for quads/triangles, strict for spheres). */
+template <bool COUNT>
+__device__ uint32_t closest_hit(const DevScene &sc, V o, V d, float &tbest, uint32_t *lds, uint32_t lane_slot,
+                                uint32_t stride, uint32_t *nodes_visited, uint32_t *prims_tested) {
+  const float tmin = EPS_ACNE;
+  float tb = FLT_MAX;
+  uint32_t best = MISS_WORD;
+  V dd = mk(fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x, fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y,
+            fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z);
+  V inv = mk(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  V oi = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+  WRay wr = wray(d);
+  uint32_t spill[STACK_SPILL];
+  int sp = 0;
+  uint32_t word = 0u; /* root node */
+  uint32_t nv = 0, np = 0;
+  const bool any_xform = sc.has_xforms != 0;
+  for (;;) {
+    if (!(word & CHILD_LEAF)) {
+      const Bvh4Node *nd = sc.nodes + word;
+      const float4 *nf = reinterpret_cast<const float4 *>(nd);
+      float4 lx = nf[0], hx = nf[1], ly = nf[2], hy = nf[3], lz = nf[4], hz = nf[5];
+      uint4 ch = reinterpret_cast<const uint4 *>(nd)[6];
+      if (COUNT) nv++;
+      float tn[4];
+      uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
+      float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+      float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+      float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+      const float tfar_lim = tb * 1.00000024f;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        float x0 = lxa[k] * inv.x - oi.x, x1 = hxa[k] * inv.x - oi.x;
+        float y0 = lya[k] * inv.y - oi.y, y1 = hya[k] * inv.y - oi.y;
+        float z0 = lza[k] * inv.z - oi.z, z1 = hza[k] * inv.z - oi.z;
+        float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+        float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
+        bool hit = (tnear <= tfar) && (cw[k] != CHILD_EMPTY);
+        tn[k] = hit ? tnear : FLT_MAX;
+      }
+      /* sort the four (t, child) pairs ascending: 5-comparator network */
+#define CSWAP(a, b)                                                       \
+  if (tn[b] < tn[a]) {                                                    \
+    float tt_ = tn[a]; tn[a] = tn[b]; tn[b] = tt_;                        \
+    uint32_t cc_ = cw[a]; cw[a] = cw[b]; cw[b] = cc_;                     \
+  }
+      CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+#undef CSWAP
+      /* push far children (hits only) so the nearest is popped first */
+#pragma unroll
+      for (int k = 3; k >= 1; k--) {
+        if (tn[k] < FLT_MAX && sp < STACK_LDS + STACK_SPILL) { /* the host bounds the depth so this always holds */
+          if (sp < STACK_LDS) lds[(uint32_t)sp * stride + lane_slot] = cw[k];
+          else spill[sp - STACK_LDS] = cw[k];
+          sp++;
+        }
+      }
+      if (tn[0] < FLT_MAX) {
+        word = cw[0];
+        continue;
+      }
+    } else {
+      uint32_t cnt = ((word >> 28) & 7u) + 1u;
+      uint32_t off = word & LEAF_OFFSET_MASK;
+      for (uint32_t k = 0; k < cnt; k++) {
+        const float4 *rec = sc.prims + off;
+        uint32_t hdr = as_u(rec[0].w);
+        float t;
+        bool h = any_xform ? test_prim<true>(sc, rec, hdr, o, d, wr, tmin, tb, t)
+                           : test_prim<false>(sc, rec, hdr, o, d, wr, tmin, tb, t);
+        if (COUNT) np++;
+        if (h) {
+          tb = t;
+          best = off;
+        }
+        off += prim_units(hdr & 3u);
+      }
+    }
+    if (sp == 0) break;
+    sp--;
+    word = sp < STACK_LDS ? lds[(uint32_t)sp * stride + lane_slot] : spill[sp - STACK_LDS];
+  }
+  if (COUNT) {
+    *nodes_visited += nv;
+    *prims_tested += np;
+  }
+  tbest = tb;
+  return best;
+}
+
+/* ------------------------------------------------------------ kernels --- */
+/* before a bounce: count the rays about to be traced, clear the shading
+ * queues and the next ray queue (a kernel, not a memset, so it can live in a
+ * captured graph) */
+__global__ void k_reset_counts(uint32_t *counts, uint32_t in_slot, uint32_t out_slot, unsigned long long *stats) {
+  uint32_t i = threadIdx.x;
+  if (i == 0) stats[2] += (unsigned long long)counts[in_slot];
+  __syncthreads();
+  if (i < MC_COUNT) counts[CNT_SHADE + i] = 0u;
+  if (i == 0) counts[out_slot] = 0u;
+}
+
+__global__ void k_set_count(uint32_t *counts, uint32_t slot, uint32_t value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counts[slot] = value;
+}
+
+/* camera sample of sampler/random.rs:20-32 and Camera::get_ray (camera.rs:103-126) */
+__global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenParams gp, uint32_t *qout) {
+  uint32_t n = gp.paths;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t j = i / gp.pixels, o = i - j * gp.pixels;
+    uint32_t r = o / gp.width, x = o - r * gp.width;
+    uint32_t y = gp.height - 1u - r;
+    uint32_t sample = gp.sample_base + j;
+    uint64_t s = pcg_for_path(gp.seed, x + y * gp.width, sample);
+    float ox = rnd(s), oy = rnd(s);
+    float lx, ly;
+    for (;;) {
+      lx = rnd_pm1(s);
+      ly = rnd_pm1(s);
+      if (lx * lx + ly * ly <= 1.0f) break;
+    }
+    (void)rnd(s); /* time: unused by the camera, drawn to keep the stream order */
+    uint32_t wl = (uint32_t)(((uint64_t)pcg_next(s) * 400ull) >> 32);
+    float u = ((float)x + ox) / (float)gp.width;
+    float v = ((float)y + oy) / (float)gp.height;
+    const DevCamera &c = sc.cam;
+    lx *= c.lens_radius;
+    ly *= c.lens_radius;
+    V cu = mk(c.u[0], c.u[1], c.u[2]), cv = mk(c.v[0], c.v[1], c.v[2]);
+    V off = lx * cu + ly * cv;
+    V org = mk(c.origin[0], c.origin[1], c.origin[2]);
+    V dir = mk(c.llc[0], c.llc[1], c.llc[2]) + u * mk(c.horizontal[0], c.horizontal[1], c.horizontal[2]) +
+            v * mk(c.vertical[0], c.vertical[1], c.vertical[2]) - org - off;
+    dir = unit(dir);
+    org = org + off;
+    pb.ray_o[i] = make_float4(org.x, org.y, org.z, 0.0f);
+    pb.ray_d[i] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    pb.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pb.rng[i] = s;
+    pb.meta[i] = wl; /* depth 0 */
+    qout[i] = i;
+  }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_extend(DevScene sc, PathBuf pb, const uint32_t *qin, const uint32_t *counts,
+                                                 uint32_t in_slot, ShadeQueues sq, uint32_t *stats) {
+  __shared__ uint32_t stack[STACK_LDS * 256];
+  const uint32_t n = counts[in_slot];
+  uint32_t nvisit = 0, ntest = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool active = i < n;
+    uint32_t path = 0, cls = MC_MISS;
+    if (active) {
+      path = qin[i];
+      float4 o4 = pb.ray_o[path], d4 = pb.ray_d[path];
+      float t;
+      uint32_t rec = closest_hit<COUNT>(sc, xyz(o4), xyz(d4), t, stack, threadIdx.x, 256u, &nvisit, &ntest);
+      pb.hit_t[path] = t;
+      pb.hit_p[path] = rec;
+      if (rec != MISS_WORD) {
+        uint32_t info = as_u(sc.prims[rec + 1u].w);
+        cls = sc.mats[sc.info[info].material].mclass;
+      }
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < MC_COUNT; c++) wave_append(active && cls == c, path, sq.q[c], sq.counts + c);
+  }
+  if (COUNT) {
+    atomicAdd((unsigned long long *)&stats[0], (unsigned long long)nvisit);
+    atomicAdd((unsigned long long *)&stats[2], (unsigned long long)ntest);
+  }
+}
+
+/* ------------------------------------------------------------ shading --- */
+struct Hit {
+  V p, n;       /* position, normal oriented against the ray */
+  bool front;
+  uint32_t material;
+};
+
+/* Reconstruct the reference's HitRecord for the hit primitive: run the wrapper
+ * chain forward, evaluate the primitive's record, then unwind RotateY
+ * (rotate.rs:115-158: rotate back, set_face_normal with the rotated ray) and
+ * Translate (translate.rs:68-77: shift back, set_face_normal with the moved ray). */
+__device__ Hit hit_record(const DevScene &sc, uint32_t off, V o, V d, float t) {
+  const float4 *rec = sc.prims + off;
+  float4 f0 = rec[0], f1 = rec[1];
+  uint32_t hdr = as_u(f0.w), type = hdr & 3u, xf = hdr >> 2;
+  Hit h;
+  h.material = sc.info[as_u(f1.w)].material;
+  V dirs[4];
+  uint32_t nops = 0;
+  const Xform *x = sc.xforms;
+  if (xf) {
+    x = &sc.xforms[xf - 1u];
+    nops = x->n_ops;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; k++) {
+      if (k < nops) {
+        const XformOp &op = x->ops[k];
+        if (op.kind == XF_TRANSLATE) {
+          o = o - mk(op.a, op.b, op.c);
+        } else {
+          float s = op.a, c = op.b;
+          o = mk(c * o.x - s * o.z, o.y, s * o.x + c * o.z);
+          d = unit(mk(c * d.x - s * d.z, d.y, s * d.x + c * d.z));
+        }
+      }
+      dirs[k] = d;
+    }
+  }
+  V p = o + t * d;
+  V outward;
+  if (type == PT_SPHERE) {
+    V c = xyz(f0);
+    outward = unit((p - c) / f1.x);
+  } else {
+    outward = xyz(rec[3]);
+  }
+  bool front = dot(d, outward) < 0.0f;
+  V n = front ? outward : -outward;
+#pragma unroll
+  for (int k = 3; k >= 0; k--) {
+    if (k < (int)nops) {
+      const XformOp &op = x->ops[k];
+      if (op.kind == XF_TRANSLATE) {
+        p = p + mk(op.a, op.b, op.c);
+      } else {
+        float s = op.a, c = op.b;
+        p = mk(c * p.x + s * p.z, p.y, -s * p.x + c * p.z);
+        n = unit(mk(c * n.x + s * n.z, n.y, -s * n.x + c * n.z));
+      }
+      front = dot(dirs[k], n) < 0.0f;
+      n = front ? n : -n;
+    }
+  }
+  h.p = p;
+  h.n = n;
+  h.front = front;
+  return h;
+}
+
+__device__ __forceinline__ V reflect(V v, V n) { return unit(v - 2.0f * dot(v, n) * n); }
+__device__ __forceinline__ V refract(V v, V n, float ratio) {
+  float ct = fminf(-dot(v, n), 1.0f);
+  V par = ratio * (v + ct * n);
+  V perp = -sqrtf(1.0f - dot(par, par)) * n;
+  return unit(par + perp);
+}
+__device__ __forceinline__ float schlick(float cosine, float idx) {
+  float r0 = (1.0f - idx) / (1.0f + idx);
+  r0 = r0 * r0;
+  return r0 + (1.0f - r0) * powf(1.0f - cosine, 5.0f);
+}
+/* ThinFilm::interference (thin_film.rs:31-54) */
+__device__ __forceinline__ float interference(const DevMaterial &m, V d, V n, int wl) {
+  float n_ratio = 1.0f / m.film_ior;
+  float c1 = -dot(d, n);
+  float s1 = sqrtf(1.0f - c1 * c1);
+  float s2 = n_ratio * s1;
+  float c2 = sqrtf(1.0f - (s2 * s2));
+  float opd = 2.0f * m.film_ior * m.film_thickness * c2;
+  float mm = opd / (float)wl;
+  return 1.0f + cosf(mm * 2.0f * PI_F);
+}
+
+/* ONB::build_from_w (onb.rs:23-40) */
+struct Onb {
+  V u, v, w;
+};
+__device__ __forceinline__ Onb onb_from_w(V w) {
+  V a = fabsf(w.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+  Onb b;
+  b.v = unit(cross(w, a));
+  b.u = unit(cross(w, b.v));
+  b.w = w;
+  return b;
+}
+__device__ __forceinline__ V onb_local(const Onb &b, V a) { return unit(a.x * b.u + a.y * b.v + a.z * b.w); }
+
+/* UnitSphere (Marsaglia 1972), as random_unit_vector (random.rs:11-15) */
+__device__ __forceinline__ V random_unit_vector(uint64_t &s) {
+  for (;;) {
+    float x1 = rnd_pm1(s), x2 = rnd_pm1(s);
+    float sum = x1 * x1 + x2 * x2;
+    if (sum >= 1.0f) continue;
+    float f = 2.0f * sqrtf(1.0f - sum);
+    return unit(mk(x1 * f, x2 * f, 1.0f - 2.0f * sum));
+  }
+}
+
+/* light pdf of one MIS leaf (quad.rs:140-165, triangle.rs:176-201, sphere.rs:133-158) */
+__device__ float light_pdf(const DevLight &L, V o, V d) {
+  if (L.type == PT_SPHERE) {
+    V c = mk(L.q[0], L.q[1], L.q[2]);
+    float r = L.u[0];
+    float t;
+    if (!sphere_t(c, r, o, d, EPS_ACNE, INFINITY, t)) return 0.0f;
+    V oc = c - o;
+    float ctm = sqrtf(1.0f - r * r / dot(oc, oc));
+    float solid = 2.0f * PI_F * (1.0f - ctm);
+    return 1.0f / solid;
+  }
+  V q = mk(L.q[0], L.q[1], L.q[2]), u = mk(L.u[0], L.u[1], L.u[2]), v = mk(L.v[0], L.v[1], L.v[2]);
+  V n = mk(L.n[0], L.n[1], L.n[2]), w = mk(L.w[0], L.w[1], L.w[2]);
+  float denom = dot(n, d);
+  if (fabsf(denom) < EPS_RECT) return 0.0f;
+  float t = (-L.n[3] - dot(n, o)) / denom;
+  if (t < EPS_ACNE || t > INFINITY) return 0.0f;
+  V p = o + t * d;
+  V ph = p - q;
+  float alpha = dot(w, cross(ph, v));
+  float beta = dot(w, cross(u, ph));
+  bool in = alpha >= 0.0f && alpha <= 1.0f && beta >= 0.0f && beta <= 1.0f;
+  if (L.type == PT_TRI) in = in && (alpha + beta <= 1.0f);
+  if (!in) return 0.0f;
+  float dist2 = t * t * dot(d, d);
+  float cosine = fabsf(dot(d, n)) / sqrtf(dot(d, d)); /* |d . oriented normal| */
+  return dist2 / (cosine * L.w[3]);
+}
+
+/* random point / direction towards one MIS leaf (quad.rs:168-172, triangle.rs:204-220, sphere.rs:160-187) */
+__device__ V light_sample(const DevLight &L, V o, uint64_t &s) {
+  if (L.type == PT_SPHERE) {
+    V c = mk(L.q[0], L.q[1], L.q[2]);
+    float r = L.u[0];
+    V off = c - o;
+    float dist2 = dot(off, off);
+    Onb b = onb_from_w(unit(off));
+    float r1 = rnd(s), r2 = rnd(s);
+    float z = 1.0f + r2 * (sqrtf(1.0f - r * r / dist2) - 1.0f);
+    float phi = 2.0f * PI_F * r1;
+    float x = cosf(phi) * sqrtf(1.0f - z * z);
+    float y = sinf(phi) * sqrtf(1.0f - z * z);
+    return onb_local(b, unit(mk(x, y, z)));
+  }
+  float a = rnd(s), bb = rnd(s);
+  if (L.type == PT_TRI && a + bb > 1.0f) {
+    a = 1.0f - a;
+    bb = 1.0f - bb;
+  }
+  V q = mk(L.q[0], L.q[1], L.q[2]), u = mk(L.u[0], L.u[1], L.u[2]), v = mk(L.v[0], L.v[1], L.v[2]);
+  V point = q + (a * u) + (bb * v);
+  return point - o;
+}
+
+/* HitablePDF over the MIS tree: sampling descends with a fair coin wherever
+ * both children exist (bvh/hitable_trait.rs:113-127); the pdf is the
+ * 2^-depth weighted sum over leaves, i.e. the nested midpoints of :89-111. */
+__device__ V tree_sample(const DevScene &sc, V o, uint64_t &s) {
+  uint32_t w = sc.light_root;
+  while (!(w & LT_LEAF)) {
+    const DevLightNode &nd = sc.lnodes[w];
+    if (nd.child[1] == LT_EMPTY) w = nd.child[0];
+    else if (nd.child[0] == LT_EMPTY) w = nd.child[1];
+    else w = coin(s) ? nd.child[0] : nd.child[1];
+  }
+  return light_sample(sc.lights[w & ~LT_LEAF], o, s);
+}
+__device__ float tree_pdf(const DevScene &sc, V o, V d) {
+  float p = 0.0f;
+  for (uint32_t l = 0; l < sc.n_lights; l++) {
+    float w = sc.lweights[l];
+    if (w > 0.0f) p += w * light_pdf(sc.lights[l], o, d);
+  }
+  return p;
+}
+
+__device__ __forceinline__ void rotate_wl(uint32_t hero_idx, int wl[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) wl[j] = CLV_LAMBDA_MIN + (int)((hero_idx + 100u * (uint32_t)j) % 400u);
+}
+
+/* One shading step of one path; returns true if the path continues. */
+template <uint32_t CLS>
+__device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint32_t i, uint32_t max_depth) {
+  uint32_t meta = pb.meta[i];
+  uint32_t hero_idx = meta & 0x1FFu;
+  uint32_t depth = meta >> 9;
+  int wl[4];
+  rotate_wl(hero_idx, wl);
+  float4 T = pb.thr[i];
+  if (CLS == MC_MISS) {
+    const float *bg = sc.spd + sc.bg_spd;
+    float4 L = pb.rad[i];
+    L.x += T.x * bg[wl[0] - CLV_LAMBDA_MIN];
+    L.y += T.y * bg[wl[1] - CLV_LAMBDA_MIN];
+    L.z += T.z * bg[wl[2] - CLV_LAMBDA_MIN];
+    L.w += T.w * bg[wl[3] - CLV_LAMBDA_MIN];
+    pb.rad[i] = L;
+    return false;
+  }
+  float4 o4 = pb.ray_o[i], d4 = pb.ray_d[i];
+  V o = xyz(o4), d = xyz(d4);
+  float t = pb.hit_t[i];
+  Hit h = hit_record(sc, pb.hit_p[i], o, d, t);
+  const DevMaterial &m = sc.mats[h.material];
+  if (CLS == MC_LIGHT) { /* DiffuseLight: emit on the front face, never scatters */
+    if (h.front) {
+      const float *e = sc.spd + m.emit_spd;
+      float4 L = pb.rad[i];
+      L.x += T.x * e[wl[0] - CLV_LAMBDA_MIN];
+      L.y += T.y * e[wl[1] - CLV_LAMBDA_MIN];
+      L.z += T.z * e[wl[2] - CLV_LAMBDA_MIN];
+      L.w += T.w * e[wl[3] - CLV_LAMBDA_MIN];
+      pb.rad[i] = L;
+    }
+    return false;
+  }
+  uint64_t s = pb.rng[i];
+  /* attenuation (trace.rs:61-70; materials.rs:90-102) */
+  const bool wdep = m.thin_film || m.kind == CLV_MAT_DISPERSIVE;
+  float att[4];
+  {
+    const float *alb = m.albedo_spd >= 0 ? sc.spd + m.albedo_spd : nullptr;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (wdep && j > 0) { att[j] = 0.0f; continue; }
+      float k = alb ? alb[wl[j] - CLV_LAMBDA_MIN] : 1.0f;
+      float tf = m.thin_film ? interference(m, d, h.n, wl[j]) : 1.0f;
+      att[j] = k * tf;
+    }
+  }
+  V nd;
+  if (CLS == MC_LAMBERTIAN) {
+    /* MixturePDF(HitablePDF(lights), CosinePDF(normal)) -- trace.rs:80-120, pdf.rs:106-141 */
+    Onb b = onb_from_w(h.n);
+    V gen;
+    if (coin(s)) {
+      gen = tree_sample(sc, h.p, s);
+    } else {
+      float r1 = rnd(s), r2 = rnd(s);
+      float z = sqrtf(1.0f - r2);
+      float phi = 2.0f * PI_F * r1;
+      gen = onb_local(b, mk(cosf(phi) * sqrtf(r2), sinf(phi) * sqrtf(r2), z));
+    }
+    nd = unit(gen);
+    float lp = tree_pdf(sc, h.p, nd);
+    V ndn = unit(nd);
+    float cs = dot(ndn, b.w);
+    float cp = cs <= 0.0f ? 0.0f : cs / PI_F;
+    float mis = 0.5f * lp + 0.5f * cp;
+    float c2 = dot(h.n, ndn);
+    if (c2 < 0.0f) { /* scattering_pdf None: return emitted (0 for Lambertian) */
+      pb.rng[i] = s;
+      return false;
+    }
+    float spdf = c2 / PI_F;
+    T.x = T.x * att[0] * spdf / mis;
+    T.y = T.y * att[1] * spdf / mis;
+    T.z = T.z * att[2] * spdf / mis;
+    T.w = T.w * att[3] * spdf / mis;
+  } else {
+    if (CLS == MC_METAL) { /* metal.rs:26-52 */
+      V r = reflect(d, h.n);
+      nd = unit(r + m.fuzz * random_unit_vector(s));
+    } else {               /* dielectric.rs:35-70, dispersive.rs:68-115 */
+      float n_ior;
+      if (m.kind == CLV_MAT_DISPERSIVE) {
+        float wm = (float)(CLV_LAMBDA_MIN + (int)hero_idx) / 1000.0f;
+        n_ior = m.cauchy_a + (m.cauchy_b / (wm * wm));
+      } else {
+        n_ior = m.ior;
+      }
+      float ratio = h.front ? 1.0f / n_ior : n_ior;
+      float ct = fminf(-dot(d, h.n), 1.0f);
+      float st = sqrtf(1.0f - ct * ct);
+      if (ratio * st > 1.0f) {
+        nd = reflect(d, h.n);
+      } else {
+        float rp = schlick(ct, ratio);
+        nd = (rnd(s) < rp) ? reflect(d, h.n) : refract(d, h.n, ratio);
+      }
+    }
+    T.x *= att[0];
+    T.y *= att[1];
+    T.z *= att[2];
+    T.w *= att[3];
+  }
+  pb.rng[i] = s;
+  if (depth + 1u > max_depth) return false; /* trace() at depth > max_depth returns 0 */
+  pb.thr[i] = T;
+  pb.ray_o[i] = make_float4(h.p.x, h.p.y, h.p.z, 0.0f);
+  pb.ray_d[i] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+  pb.meta[i] = hero_idx | ((depth + 1u) << 9);
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQueues sq, uint32_t *qout,
+                                                uint32_t *counts, uint32_t out_slot, uint32_t max_depth) {
+  uint32_t c[MC_COUNT], chunks[MC_COUNT + 1];
+  chunks[0] = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < MC_COUNT; k++) {
+    c[k] = sq.counts[k];
+    chunks[k + 1] = chunks[k] + (c[k] + 63u) / 64u;
+  }
+  const uint32_t total = chunks[MC_COUNT];
+  const uint32_t waves_per_block = blockDim.x / 64u;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t wv = blockIdx.x * waves_per_block + threadIdx.x / 64u; __builtin_amdgcn_readfirstlane(wv) < total;
+       wv += gridDim.x * waves_per_block) {
+    uint32_t cls = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < MC_COUNT; k++)
+      if (wv >= chunks[k]) cls = k;
+    cls = __builtin_amdgcn_readfirstlane(cls);
+    uint32_t idx = (wv - chunks[cls]) * 64u + lane;
+    bool active = idx < c[cls];
+    uint32_t path = active ? sq.q[cls][idx] : 0u;
+    bool cont = false;
+    if (active) {
+      switch (cls) {
+        case MC_MISS: cont = shade_path<MC_MISS>(sc, pb, path, max_depth); break;
+        case MC_LIGHT: cont = shade_path<MC_LIGHT>(sc, pb, path, max_depth); break;
+        case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN>(sc, pb, path, max_depth); break;
+        case MC_METAL: cont = shade_path<MC_METAL>(sc, pb, path, max_depth); break;
+        default: cont = shade_path<MC_GLASS>(sc, pb, path, max_depth); break;
+      }
+    }
+    wave_append(cont, path, qout, counts + out_slot);
+  }
+}
+
+/* per-sample estimator (draw_cpu.rs:130-142) and per-pixel sums in sample order */
+__global__ __launch_bounds__(256) void k_resolve(PathBuf pb, ResolveParams rp, const float4 *cie) {
+  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < rp.pixels; o += gridDim.x * blockDim.x) {
+    float a32[3];
+    double a64[3], sq[3];
+    for (int k = 0; k < 3; k++) {
+      sq[k] = rp.acc_sq[3 * o + k];
+      if (rp.fp64) a64[k] = rp.acc64[3 * o + k];
+      else a32[k] = rp.acc32[3 * o + k];
+    }
+    for (uint32_t j = 0; j < rp.per_pixel; j++) {
+      uint32_t i = j * rp.pixels + o;
+      float4 L = pb.rad[i];
+      uint32_t hero = pb.meta[i] & 0x1FFu;
+      float c[3] = {0.0f, 0.0f, 0.0f};
+      if (L.y == 0.0f && L.z == 0.0f && L.w == 0.0f) {
+        float4 x = cie[hero];
+        c[0] = x.x * L.x;
+        c[1] = x.y * L.x;
+        c[2] = x.z * L.x;
+      } else {
+        float Ls[4] = {L.x, L.y, L.z, L.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          float p = Ls[q];
+          if (isnormal(p) && !signbit(p)) {
+            float4 x = cie[(hero + 100u * (uint32_t)q) % 400u];
+            c[0] += x.x * p / 4.0f;
+            c[1] += x.y * p / 4.0f;
+            c[2] += x.z * p / 4.0f;
+          }
+        }
+      }
+      for (int k = 0; k < 3; k++) {
+        if (rp.fp64) a64[k] += (double)c[k];
+        else a32[k] += c[k];
+        sq[k] += (double)c[k] * (double)c[k];
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      rp.acc_sq[3 * o + k] = sq[k];
+      if (rp.fp64) rp.acc64[3 * o + k] = a64[k];
+      else rp.acc32[3 * o + k] = a32[k];
+    }
+  }
+}
+
+__global__ void k_finalize(ResolveParams rp, float *out, float *out_sq, uint32_t spp) {
+  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < rp.pixels * 3u; o += gridDim.x * blockDim.x) {
+    out[o] = rp.fp64 ? (float)(rp.acc64[o] / (double)spp) : rp.acc32[o] / (float)spp;
+    if (out_sq) out_sq[o] = (float)(rp.acc_sq[o] / (double)spp);
+  }
+}
+
+__global__ void k_zero_accum(ResolveParams rp) {
+  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < rp.pixels * 3u; o += gridDim.x * blockDim.x) {
+    rp.acc_sq[o] = 0.0;
+    rp.acc64[o] = 0.0;
+    rp.acc32[o] = 0.0f;
+  }
+}
+
+/* primary-ray probe: closest hit of the camera rays written by k_generate */
+__global__ __launch_bounds__(256) void k_primary(DevScene sc, PathBuf pb, uint32_t n, uint32_t *ids, float *ts) {
+  __shared__ uint32_t stack[STACK_LDS * 256];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    if (i >= n) continue;
+    float t;
+    uint32_t rec = closest_hit<false>(sc, xyz(pb.ray_o[i]), xyz(pb.ray_d[i]), t, stack, threadIdx.x, 256u, nullptr,
+                                      nullptr);
+    if (rec == MISS_WORD) {
+      ids[i] = 0xFFFFFFFFu;
+      ts[i] = INFINITY;
+    } else {
+      ids[i] = sc.info[as_u(sc.prims[rec + 1u].w)].report_id;
+      ts[i] = t;
+    }
+  }
+}
+
+/* ---------------------------------------------------------- launchers --- */
+static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
+  uint32_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return g < cap ? g : cap;
+}
+
+void launch_reset_counts(hipStream_t st, uint32_t *counts, uint32_t in_slot, uint32_t out_slot, uint64_t *stats) {
+  hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, counts, in_slot, out_slot, (unsigned long long *)stats);
+}
+void launch_set_count(hipStream_t st, uint32_t *counts, uint32_t slot, uint32_t value) {
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, st, counts, slot, value);
+}
+void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp, uint32_t *qout) {
+  hipLaunchKernelGGL(k_generate, dim3(grid_for(gp.paths, 256, 65535)), dim3(256), 0, st, sc, pb, gp, qout);
+}
+void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const uint32_t *qin, const uint32_t *counts,
+                   uint32_t in_slot, const ShadeQueues &sq, uint32_t max_paths, uint32_t grid_cap, uint64_t *stats) {
+  uint32_t g = grid_for(max_paths, 256, grid_cap);
+  if (stats)
+    hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(256), 0, st, sc, pb, qin, counts, in_slot, sq, (uint32_t *)stats);
+  else
+    hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(256), 0, st, sc, pb, qin, counts, in_slot, sq, nullptr);
+}
+void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
+                  uint32_t *counts, uint32_t out_slot, uint32_t max_depth, uint32_t max_paths, uint32_t grid_cap) {
+  uint32_t g = grid_for(max_paths + 64u * MC_COUNT, 256, grid_cap);
+  hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, max_depth);
+}
+void launch_resolve(hipStream_t st, const PathBuf &pb, const ResolveParams &rp, const float4 *cie) {
+  hipLaunchKernelGGL(k_resolve, dim3(grid_for(rp.pixels, 256, 65535)), dim3(256), 0, st, pb, rp, cie);
+}
+void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float *out_sq, uint32_t spp) {
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(rp.pixels * 3u, 256, 65535)), dim3(256), 0, st, rp, out, out_sq, spp);
+}
+void launch_zero_accum(hipStream_t st, const ResolveParams &rp) {
+  hipLaunchKernelGGL(k_zero_accum, dim3(grid_for(rp.pixels * 3u, 256, 65535)), dim3(256), 0, st, rp);
+}
+void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, uint32_t n, uint32_t *ids, float *ts) {
+  hipLaunchKernelGGL(k_primary, dim3(grid_for(n, 256, 65535)), dim3(256), 0, st, sc, pb, n, ids, ts);
+}
+
+}  // namespace clv

@@ -1,0 +1,247 @@
+/*
+ * clovers_hip.h -- C ABI of the MI355X wavefront spectral path tracer.
+ *
+ * This is the drop-in boundary for the reference renderer's hot path.  The
+ * reference (Walther/clovers, Rust) renders with exactly one call,
+ *
+ *     draw_cpu::draw(&GlobalOptions, &RenderOptions, &Scene, Sampler) -> Vec<Xyz<E>>
+ *         clovers-cli/src/draw_cpu.rs:26-98, called at clovers-cli/src/render.rs:149
+ *
+ * after building a `Scene` (clovers/src/scenes.rs:9-18) from a scene file
+ * (clovers-cli/src/scenefile.rs:34-117).  Here the scene arrives as a flat,
+ * plain-old-data descriptor (clv_scene_desc) that mirrors the reference's
+ * object/material model one-to-one, and clv_render() returns the same pixel
+ * buffer draw() returns: linear XYZ with white point E, row 0 = top of the
+ * image, index = x + row*W, averaged over samples (draw_cpu.rs:51,66,145).
+ *
+ * Conventions: C99, POD structs only, no exceptions cross the boundary, every
+ * entry point returns a clv_status and never aborts.  A context is not
+ * re-entrant (calls on one clv_ctx are serialised by the caller); separate
+ * contexts may be used from separate host threads.  All device memory is
+ * owned by the context.  All pointers passed in are host pointers; the library
+ * copies what it keeps.
+ */
+#ifndef CLOVERS_HIP_H
+#define CLOVERS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLV_ABI_VERSION 1u
+
+typedef enum {
+  CLV_OK = 0,
+  CLV_EINVAL = 1,        /* malformed argument or scene */
+  CLV_ENOMEM = 2,        /* host or device allocation failed */
+  CLV_EDEVICE = 3,       /* no HIP device / HIP runtime error */
+  CLV_EUNSUPPORTED = 4,  /* valid in the reference's schema, not supported here */
+  CLV_EINTERNAL = 5
+} clv_status;
+
+/* ---------------------------------------------------------------- scene --- */
+
+/* Object kinds: clovers/src/objects.rs:59-87 (the in-scope subset).  A Boxy is
+ * one object made of six quads (objects/boxy.rs:43-83). */
+typedef enum {
+  CLV_OBJ_SPHERE = 0,   /* p = center.xyz, radius                         (objects/sphere.rs:43-54) */
+  CLV_OBJ_QUAD = 1,     /* p = q.xyz, u.xyz, v.xyz                        (objects/quad.rs:248-270) */
+  CLV_OBJ_TRIANGLE = 2, /* p = a.xyz, b.xyz, c.xyz (absolute corners)     (objects/triangle.rs:106-117) */
+  CLV_OBJ_BOXY = 3,     /* p = corner_0.xyz, corner_1.xyz                  (objects/boxy.rs:43-83) */
+  CLV_OBJ_TRIANGLE_QUV = 4 /* p = q.xyz, u.xyz, v.xyz (scene-file Triangle)  (objects/triangle.rs:61-101) */
+} clv_object_kind;
+
+/* One top-level hitable of the reference's flattened list (scenefile.rs:63-103):
+ * STL/PLY/ObjectList lists are flattened into one object per element. */
+typedef struct {
+  uint32_t kind;      /* clv_object_kind */
+  uint32_t material;  /* index into clv_scene_desc.materials */
+  uint32_t priority;  /* 1: also a multiple-importance-sampling target (scenefile.rs:89-100) */
+  uint32_t xform;     /* index into clv_scene_desc.xforms, or CLV_NO_XFORM */
+  float p[9];
+  uint32_t _pad[3];
+} clv_object;         /* 64 bytes */
+
+#define CLV_NO_XFORM 0xFFFFFFFFu
+
+/* Instance wrappers: Translate (objects/translate.rs:41-103) and RotateY
+ * (objects/rotate.rs:40-182).  ops[0] is the OUTERMOST wrapper, e.g. the
+ * reference idiom Translate{RotateY{Boxy}} is {TRANSLATE, ROTATE_Y}. */
+typedef enum { CLV_XF_TRANSLATE = 0, CLV_XF_ROTATE_Y = 1 } clv_xform_op_kind;
+#define CLV_MAX_XFORM_OPS 4
+typedef struct {
+  uint32_t kind;      /* clv_xform_op_kind */
+  float v[3];         /* TRANSLATE: offset.xyz; ROTATE_Y: angle in degrees in v[0] */
+} clv_xform_op;
+typedef struct {
+  uint32_t n_ops;
+  uint32_t _pad[3];
+  clv_xform_op ops[CLV_MAX_XFORM_OPS];
+} clv_xform;
+
+/* Material kinds: clovers/src/materials.rs:141-156 (in-scope subset). */
+typedef enum {
+  CLV_MAT_LAMBERTIAN = 0,     /* materials/lambertian.rs  albedo texture */
+  CLV_MAT_METAL = 1,          /* materials/metal.rs       albedo texture, fuzz */
+  CLV_MAT_DIELECTRIC = 2,     /* materials/dielectric.rs  refractive_index, color (Xyz<E>) */
+  CLV_MAT_DISPERSIVE = 3,     /* materials/dispersive.rs  cauchy_a, cauchy_b (um^2) */
+  CLV_MAT_DIFFUSE_LIGHT = 4   /* materials/diffuse_light.rs  emit texture */
+} clv_material_kind;
+
+/* Texture kinds: clovers/src/textures.rs:22-109 (in-scope subset). */
+typedef enum {
+  CLV_TEX_SOLID = 0,          /* SolidColor: color = Xyz<E> (textures/solid_color.rs) */
+  CLV_TEX_ILLUMINANT_D65 = 1, /* CIE D65 table x intensity (build.rs:66-112) */
+  CLV_TEX_ILLUMINANT_D50 = 2  /* CIE D50 table x intensity */
+} clv_texture_kind;
+
+typedef struct {
+  uint32_t kind;          /* clv_material_kind */
+  uint32_t texture;       /* clv_texture_kind of albedo (Lambertian/Metal) or emit (DiffuseLight) */
+  float color[3];         /* SolidColor / Dielectric color, already converted to Xyz<E> */
+  float intensity;        /* illuminant intensity (default 1.0) */
+  float fuzz;             /* Metal; NOT clamped (the scene-file path does not clamp, metal.rs:60) */
+  float refractive_index; /* Dielectric (default 1.5) */
+  float cauchy_a;         /* Dispersive (default 1.5046) */
+  float cauchy_b;         /* Dispersive (default 0.00420) */
+  uint32_t thin_film;     /* 1: ThinFilm layer present (materials.rs:63-103, thin_film.rs) */
+  float film_index;       /* ThinFilm refractive_index (default 1.5) */
+  float film_thickness;   /* ThinFilm thickness in nm (default 500) */
+  uint32_t _pad;
+} clv_material;           /* 56 bytes */
+
+/* CameraInit + scene times (camera.rs:40-54, scenefile.rs:21-29). */
+typedef struct {
+  float look_from[3];
+  float look_at[3];
+  float up[3];
+  float vertical_fov;     /* degrees */
+  float aperture;
+  float focus_distance;
+  float time_0, time_1;
+} clv_camera_init;
+
+typedef enum { CLV_BVH_LAM = 0, CLV_BVH_SAH = 1 } clv_bvh_algorithm; /* bvh.rs:32-40 */
+
+typedef struct {
+  uint32_t abi_version;           /* must be CLV_ABI_VERSION */
+  uint32_t n_objects;
+  const clv_object *objects;      /* in scene-file order (affects only BVH shape) */
+  uint32_t n_materials;
+  const clv_material *materials;
+  uint32_t n_xforms;
+  const clv_xform *xforms;
+  clv_camera_init camera;
+  float background[3];            /* Xyz<E> of the scene background (scenefile.rs:43-44) */
+  uint32_t mis_bvh_algorithm;     /* clv_bvh_algorithm of the MIS tree (scenefile.rs:108) */
+} clv_scene_desc;
+
+/* --------------------------------------------------------------- render --- */
+
+typedef enum {
+  CLV_GUIDE_OFF = 0,      /* plain method: the reference's light/BSDF mixture (trace.rs:80-120) */
+  CLV_GUIDE_ON = 1        /* learned vMF-mixture proposal, one-sample MIS with the plain mixture */
+} clv_guiding;
+
+/* RenderOptions (clovers-cli/src/render.rs:17-49) plus the seed that replaces
+ * the reference's OS-seeded per-row generators (draw_cpu.rs:56,62). */
+typedef struct {
+  uint32_t width, height;
+  uint32_t samples;        /* spp */
+  uint32_t max_depth;      /* trace() returns 0 once depth > max_depth (trace.rs:35) */
+  uint64_t seed;           /* per-path PCG streams are keyed by (seed, pixel, sample) */
+  uint32_t sample_offset;  /* first sample index (progressive rendering / sharding by sample) */
+  uint32_t guiding;        /* clv_guiding */
+  uint32_t fp64_accum;     /* 1: accumulate per-pixel sums in f64 */
+  uint32_t flags;          /* CLV_FLAG_* */
+} clv_render_opts;
+
+#define CLV_FLAG_TIME_KERNELS 1u  /* time every traversal launch with HIP events (clv_counters.extend_ms) */
+#define CLV_FLAG_COUNT 2u         /* counting traversal: node visits / primitive tests (slower) */
+
+/* Counters of the last clv_render / clv_primary_hits on a context. */
+typedef struct {
+  uint64_t samples;          /* camera paths traced */
+  uint64_t rays;             /* extension rays traced (all bounces) */
+  uint64_t node_visits;      /* wide-BVH nodes fetched (CLV_FLAG_COUNT only, else 0) */
+  uint64_t prim_tests;       /* primitives tested (CLV_FLAG_COUNT only, else 0) */
+  double render_ms;          /* device time of the render, HIP events on the context's stream */
+  double extend_ms;          /* summed device time of the traversal launches (CLV_FLAG_TIME_KERNELS) */
+  double build_ms;           /* host time of the last clv_scene_upload (spectra + BVHs) */
+  uint32_t extend_launches;  /* traversal launches of the render */
+  uint32_t passes;           /* wavefront passes (each covers a block of samples of every pixel) */
+  uint32_t bvh_nodes;        /* wide nodes of the scene BVH */
+  uint32_t bvh_width;        /* children per node */
+  uint32_t node_bytes;       /* bytes per wide node */
+  uint32_t prim_bytes;       /* bytes of primitive records / number of primitives */
+  uint32_t n_prims;          /* primitives after flattening (a Boxy is 6) */
+  uint32_t bvh_depth;        /* depth of the wide BVH */
+} clv_counters;
+
+typedef struct clv_ctx clv_ctx;
+
+/* Create a context on HIP device `hip_device` (binds the device and a stream). */
+clv_status clv_create(int hip_device, clv_ctx **out);
+/* Free every device allocation of the context. NULL is accepted. */
+void clv_destroy(clv_ctx *ctx);
+/* Human-readable detail of the last failure on ctx (valid until the next call). */
+const char *clv_last_error(const clv_ctx *ctx);
+
+/* Build (host: spectra, instance chains, BVH) and upload a scene.
+ * Replaces SceneFile::initialize's BVH builds (scenefile.rs:104-109).
+ * The scene stays resident in HBM until clv_scene_free or clv_destroy. */
+clv_status clv_scene_upload(clv_ctx *ctx, const clv_scene_desc *desc, uint32_t *scene_id);
+clv_status clv_scene_free(clv_ctx *ctx, uint32_t scene_id);
+
+/* Host-only scene check, no device needed: runs the full host build (spectra,
+ * flattening, scene BVH, light tree) and reports its shape.  The counterpart of
+ * the reference's `validate` subcommand (clovers-cli/src/validate.rs:14-30),
+ * which initialises a scene and reports success or the failure. */
+typedef struct {
+  uint32_t n_objects;
+  uint32_t n_prims;        /* after flattening (a Boxy is 6 quads) */
+  uint32_t n_lights;       /* multiple-importance-sampling targets */
+  uint32_t bvh_nodes;      /* 4-wide nodes */
+  uint32_t bvh_depth;
+  uint32_t light_nodes;    /* binary light-tree nodes */
+  uint64_t device_bytes;   /* HBM the scene occupies once uploaded */
+  double build_ms;
+  char error[256];         /* empty on success */
+} clv_scene_info;
+clv_status clv_scene_validate(const clv_scene_desc *desc, clv_scene_info *info);
+
+/* == draw_cpu::draw for RenderMode::PathTracing (draw_cpu.rs:26-146).
+ * xyz_out: caller-allocated W*H*3 floats (row 0 = top, index = x + row*W),
+ * the per-pixel mean over opts->samples.  xyz_sq_out (nullable): per-pixel
+ * mean of the squared per-sample contribution, for Monte Carlo error bars.
+ * Blocking. */
+clv_status clv_render(clv_ctx *ctx, uint32_t scene_id, const clv_render_opts *opts,
+                      float *xyz_out, float *xyz_sq_out);
+
+/* Primary-ray closest hits for sample `opts->sample_offset` of every pixel,
+ * same camera and ray seeds as clv_render (the parity probe of the hit path).
+ * prim_id_out: W*H ids, object_index*8 + face (face = 0..5 for a Boxy side,
+ * else 0), 0xFFFFFFFF on a miss.  t_out: W*H hit distances (nullable). */
+clv_status clv_primary_hits(clv_ctx *ctx, uint32_t scene_id, const clv_render_opts *opts,
+                            uint32_t *prim_id_out, float *t_out);
+
+/* Counters of the last render on ctx. */
+clv_status clv_counters_get(const clv_ctx *ctx, clv_counters *out);
+
+/* Library ABI version (CLV_ABI_VERSION the library was built with). */
+uint32_t clv_abi_version(void);
+
+/* Host-side spectral helpers, no device needed.
+ * clv_spectral_power: clovers::spectrum::spectral_power (clovers/src/spectrum.rs:46-56),
+ *   Meng et al. 2015 upsampling of an Xyz<E> colour at an integer wavelength in nm.
+ * clv_wavelength_into_xyz: clovers::wavelength::wavelength_into_xyz
+ *   (clovers/src/wavelength.rs:62-83) for lambda in [380, 780); CLV_EINVAL outside. */
+float clv_spectral_power(const float xyz[3], uint32_t lambda);
+clv_status clv_wavelength_into_xyz(uint32_t lambda, float xyz_out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLOVERS_HIP_H */

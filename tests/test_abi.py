@@ -1,0 +1,66 @@
+"""The C-ABI library loads (no device needed) and exports exactly what
+include/clovers_hip.h declares; the Python ctypes mirror has the C layout."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+import clovers_amd
+from clovers_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "clovers_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(clv_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = clovers_amd.load_library()
+    names = declared_functions()
+    assert "clv_render" in names and "clv_primary_hits" in names and len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.clv_abi_version() == abi.ABI_VERSION
+
+
+def test_struct_layouts_match_the_header():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "clovers_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(clv_object), sizeof(clv_material), sizeof(clv_xform),
+         sizeof(clv_camera_init), sizeof(clv_scene_desc), sizeof(clv_render_opts), sizeof(clv_counters),
+         sizeof(clv_scene_info), offsetof(clv_scene_desc, camera));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        got = [int(v) for v in subprocess.check_output([exe]).split()]
+    want = [abi.OBJECT_DTYPE.itemsize, abi.MATERIAL_DTYPE.itemsize, abi.XFORM_DTYPE.itemsize,
+            C.sizeof(abi.CameraInit), C.sizeof(abi.SceneDesc), C.sizeof(abi.RenderOpts), C.sizeof(abi.Counters),
+            C.sizeof(abi.SceneInfo), abi.SceneDesc.camera.offset]
+    assert got == want
+
+
+def test_no_device_calls_fail_cleanly_without_gpu():
+    """Null arguments are rejected before any device work."""
+    lib = clovers_amd.load_library()
+    assert lib.clv_render(None, 1, None, None, None) == abi.EINVAL
+    assert lib.clv_primary_hits(None, 1, None, None, None) == abi.EINVAL
+    assert lib.clv_counters_get(None, None) == abi.EINVAL
+    assert lib.clv_scene_validate(None, None) == abi.EINVAL
+    out = (C.c_float * 3)()
+    assert lib.clv_wavelength_into_xyz(379, out) == abi.EINVAL
+    assert lib.clv_wavelength_into_xyz(780, out) == abi.EINVAL

@@ -1,0 +1,128 @@
+"""Geometry of the oracle against the reference's known-answer tests.
+
+* clovers/src/aabb.rs:180-283: areas, centroids, combine;
+* clovers/src/interval.rs:64-100: center/size (through the box centroid);
+* clovers/src/objects/triangle.rs:237-417: the four unit-triangle fixtures
+  (padded AABB, box hit, t = 1, position, normal (0,0,-1), front_face);
+* closed forms for spheres and quads (brute-force intersection in float64).
+"""
+import numpy as np
+import pytest
+
+from oracle import py_oracle as po
+
+EPS_RECT = np.float32(0.0001)
+
+
+def box(x0, x1, y0, y1, z0, z1):
+    return [x0, x1, y0, y1, z0, z1]
+
+
+def test_aabb_area_known_answers():
+    assert po.aabb_area(box(0, 1, 0, 1, 0, 1)) == 6.0
+    assert po.aabb_area(box(0, 1, 0, 2, 0, 3)) == 22.0
+    assert po.aabb_area(box(-1, 0, -2, 0, -3, 0)) == 22.0
+
+
+def test_aabb_centroid_and_interval_center():
+    assert list(po.aabb_centroid(box(0, 1, 0, 1, 0, 1))) == [0.5, 0.5, 0.5]
+    assert list(po.aabb_centroid(box(0, 0, 0, 0, 0, 0))) == [0.0, 0.0, 0.0]
+    assert list(po.aabb_centroid(box(-1, 1, -1, 1, -1, 1))) == [0.0, 0.0, 0.0]
+
+
+def test_aabb_combine_known_answers():
+    z = box(0, 0, 0, 0, 0, 0)
+    u = box(0, 1, 0, 1, 0, 1)
+    n = box(-1, 0, -1, 0, -1, 0)  # Interval::new(0,-1) orders its ends
+    assert list(po.aabb_combine(z, z)) == z
+    assert list(po.aabb_combine(z, u)) == u
+    assert list(po.aabb_combine(u, n)) == box(-1, 1, -1, 1, -1, 1)
+
+
+RAY_POS = ((0.01, 0.01, -1.0), (0.0, 0.0, 1.0))
+RAY_NEG = ((-0.01, -0.01, -1.0), (0.0, 0.0, 1.0))
+
+
+@pytest.mark.parametrize("u,v,ray,box_xy,front", [
+    ((1, 0, 0), (0, 1, 0), RAY_POS, (0, 1), False),     # xy_unit_triangle
+    ((0, 1, 0), (1, 0, 0), RAY_POS, (0, 1), True),      # yx_unit_triangle
+    ((-1, 0, 0), (0, -1, 0), RAY_NEG, (-1, 0), False),  # neg_xy_unit_triangle
+    ((0, -1, 0), (-1, 0, 0), RAY_NEG, (-1, 0), True),   # neg_yx_unit_triangle
+])
+def test_triangle_fixtures(u, v, ray, box_xy, front):
+    o, d = ray
+    hit, out = po.triangle_probe((0, 0, 0), u, v, o, d, -np.inf, np.inf)
+    lo, hi = box_xy
+    ez = np.float32(0.0) - EPS_RECT, np.float32(0.0) + EPS_RECT
+    assert list(out[:6]) == [lo, hi, lo, hi, ez[0], ez[1]]
+    assert po.aabb_hit(out[:6], o, d, 0.0, 1.0)
+    assert hit
+    assert out[6] - 1.0 <= np.finfo(np.float32).eps
+    assert list(out[7:10]) == [o[0], o[1], 0.0]
+    assert list(out[10:13]) == [0.0, 0.0, -1.0]
+    assert bool(out[13]) == front
+
+
+def test_triangle_rejects_outside_and_parallel():
+    hit, _ = po.triangle_probe((0, 0, 0), (1, 0, 0), (0, 1, 0), (0.6, 0.6, -1), (0, 0, 1), 0.001, np.inf)
+    assert not hit  # alpha + beta > 1
+    hit, _ = po.triangle_probe((0, 0, 0), (1, 0, 0), (0, 1, 0), (0.1, 0.1, -1), (1, 0, 0), 0.001, np.inf)
+    assert not hit  # parallel ray
+
+
+def test_sampler_distributions():
+    """UnitSphere (Marsaglia 1972), UnitDisc (rejection) and cosine-weighted
+    hemisphere samples: unit length / inside the disc, and chi-square tests of
+    their distributions."""
+    from scipy import stats
+    n = 200_000
+    s = po.sample_probe(0, n, seed=11).astype(np.float64)
+    assert np.allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-6)
+    # z uniform in [-1,1] (Archimedes) and phi uniform
+    for col in (s[:, 2], (np.arctan2(s[:, 1], s[:, 0]) + np.pi) / (2 * np.pi) * 2 - 1):
+        h, _ = np.histogram(col, bins=40, range=(-1, 1))
+        assert stats.chisquare(h).pvalue > 1e-4
+    d = po.sample_probe(1, n, seed=12).astype(np.float64)
+    r2 = d[:, 0] ** 2 + d[:, 1] ** 2
+    assert np.all(r2 <= 1.0)
+    h, _ = np.histogram(r2, bins=40, range=(0, 1))       # r^2 uniform for a uniform disc
+    assert stats.chisquare(h).pvalue > 1e-4
+    c = po.sample_probe(2, n, seed=13).astype(np.float64)
+    assert np.all(c[:, 2] >= 0)
+    h, _ = np.histogram(c[:, 2] ** 2, bins=40, range=(0, 1))  # cos^2 uniform for cosine weighting
+    assert stats.chisquare(h).pvalue > 1e-4
+
+
+def _light_scene(objs):
+    from clovers_amd.scenefile import load_scene
+    return load_scene({
+        "time_0": 0, "time_1": 1,
+        "camera": {"look_from": [0, 0, 5], "look_at": [0, 0, 0], "up": [0, 1, 0], "vertical_fov": 40,
+                   "aperture": 0, "focus_distance": 10},
+        "background_color": [0, 0, 0], "objects": objs,
+        "materials": [{"name": "lamp", "kind": "DiffuseLight"}]})
+
+
+@pytest.mark.parametrize("objs", [
+    [{"kind": "Quad", "priority": True, "q": [-1, 2, -1], "u": [2, 0, 0], "v": [0, 0, 2], "material": "lamp"}],
+    [{"kind": "Sphere", "priority": True, "center": [0.5, 3, 0], "radius": 0.7, "material": "lamp"}],
+    [{"kind": "Triangle", "priority": True, "q": [-1, 2, -1], "u": [2, 0, 0], "v": [0, 0.5, 2], "material": "lamp"},
+     {"kind": "Quad", "priority": True, "q": [2, -1, -1], "u": [0, 2, 0], "v": [0, 0, 2], "material": "lamp"},
+     {"kind": "Sphere", "priority": True, "center": [-3, 0, 0], "radius": 0.5, "material": "lamp"}],
+])
+def test_light_pdf_normalised(objs):
+    """HitablePDF over the MIS tree: E_sample[1/pdf] = total solid angle
+    covered and pdf integrates to 1 over directions it can generate
+    (checked as E_uniform-sphere[pdf] * 4 pi = 1)."""
+    sc = po.OracleScene(_light_scene(objs))
+    origin = np.array([0.0, 0.0, 0.0], np.float32)
+    rng = np.random.default_rng(5)
+    n = 200_000
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    pdf = np.array([sc.light_pdf(origin, d.astype(np.float32)) for d in v.astype(np.float32)]) * 4 * np.pi
+    est, se = pdf.mean(), pdf.std() / np.sqrt(n)
+    assert abs(est - 1.0) < 4 * se + 1e-3, (est, se)
+    d, p = sc.light_probe(origin, 20000, seed=3)
+    assert np.all(p > 0)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)

@@ -1,0 +1,197 @@
+"""HIP renderer vs the oracle (the CPU restatement of the reference's hot path),
+called through the C ABI.
+
+* primary hits: same camera and ray seeds; >= 99.99 % of primitive ids equal
+  and every mismatch a t-tie within 1e-5 relative;
+* same streams: with identical per-path random streams most paths are the
+  same path, so most pixels agree to float rounding;
+* independent streams: per-block means agree within their Monte Carlo error
+  (z-scores from the per-pixel second moments);
+* closed forms (emitter seen directly, white furnace) and invariants
+  (determinism, sample-offset splitting, depth cut-off).
+"""
+import numpy as np
+import pytest
+
+from clovers_amd import abi, synth
+from oracle import py_oracle as po
+from special_scenes import direct_emitter, expected_pixel, furnace, sphere_mask
+
+pytestmark = pytest.mark.gpu
+
+SMALL_MESH = lambda: synth.mesh(freq=24)  # 11,520 noise-displaced triangles, thin-film glass
+
+
+SCENES = {
+    "cornell": (synth.cornell, 256, 256, 8),
+    "spheres": (synth.spheres, 320, 180, 16),
+    "prism": (synth.prism, 192, 192, 24),
+    "mesh": (SMALL_MESH, 192, 108, 12),
+    "furnace": (furnace, 64, 64, 6),
+}
+
+_cache = {}
+
+
+def scene(name):
+    if name not in _cache:
+        _cache[name] = SCENES[name][0]()
+    return _cache[name]
+
+
+def oracle_scene(name):
+    key = name + ":oracle"
+    if key not in _cache:
+        _cache[key] = po.OracleScene(scene(name))
+    return _cache[key]
+
+
+def gpu_scene(renderer, name):
+    key = name + ":gpu%d" % id(renderer)
+    if key not in _cache:
+        _cache[key] = renderer.upload(scene(name))
+    return _cache[key]
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_primary_hit_ids(renderer, name):
+    _, W, H, depth = SCENES[name]
+    o = abi.render_opts(W, H, 1, depth, seed=1)
+    gi, gt = renderer.primary_hits(gpu_scene(renderer, name), o)
+    oi, ot = oracle_scene(name).primary_hits(o)
+    same = gi == oi
+    frac = same.mean()
+    mism = ~same
+    both_hit = (gi != abi.MISS) & (oi != abi.MISS)
+    tie = both_hit & (np.abs(gt - ot) <= 1e-5 * np.maximum(np.abs(ot), 1e-30))
+    print("%s: %d rays, %.6f equal, %d mismatches (%d t-ties)" % (name, gi.size, frac, mism.sum(), (mism & tie).sum()))
+    assert frac >= 0.9999
+    assert np.all(tie[mism]), list(zip(gi[mism & ~tie][:5], oi[mism & ~tie][:5], gt[mism & ~tie][:5], ot[mism & ~tie][:5]))
+    hit = same & (gi != abi.MISS)
+    assert np.allclose(gt[hit], ot[hit], rtol=1e-5)
+
+
+@pytest.mark.parametrize("name", ["cornell", "spheres", "prism", "mesh"])
+def test_same_streams_same_paths(renderer, name):
+    _, W, H, depth = SCENES[name]
+    o = abi.render_opts(W, H, 1, depth, seed=7)
+    g, _ = renderer.render(gpu_scene(renderer, name), o)
+    c, _, _ = oracle_scene(name).render(o)
+    close = np.all(np.abs(g - c) <= 1e-3 * np.abs(c) + 1e-6, axis=-1)
+    print("%s: %.4f of pixels equal to float rounding" % (name, close.mean()))
+    assert close.mean() > 0.95
+
+
+def block_z(g, gsq, c, csq, n_g, n_c, b):
+    """z-scores of b x b block means of two independent estimates."""
+    H, W, _ = g.shape
+    H, W = H - H % b, W - W % b
+
+    def blk(a):
+        return a[:H, :W].reshape(H // b, b, W // b, b, 3).mean((1, 3))
+    var_g = np.maximum(gsq - g.astype(np.float64) ** 2, 0)
+    var_c = np.maximum(csq - c.astype(np.float64) ** 2, 0)
+    se2 = blk(var_g) / (n_g * b * b) + blk(var_c) / (n_c * b * b)
+    d = blk(g.astype(np.float64)) - blk(c.astype(np.float64))
+    ok = se2 > 0
+    return d[ok] / np.sqrt(se2[ok]), d, se2
+
+
+@pytest.mark.parametrize("name,spp", [("cornell", 64), ("spheres", 32), ("prism", 64), ("mesh", 32), ("furnace", 64)])
+def test_independent_streams_agree_within_error(renderer, name, spp):
+    _, W, H, depth = SCENES[name]
+    og = abi.render_opts(W, H, spp, depth, seed=1001)
+    oc = abi.render_opts(W, H, spp, depth, seed=2002)
+    g, gsq = renderer.render(gpu_scene(renderer, name), og, want_sq=True)
+    c, csq, _ = oracle_scene(name).render(oc, want_sq=True)
+    z, d, se2 = block_z(g, gsq, c, csq, spp, spp, 8)
+    print("%s: %d block-channels, mean z^2 %.3f, max |z| %.2f, frac |z|>3 %.4f"
+          % (name, z.size, np.mean(z * z), np.abs(z).max(), np.mean(np.abs(z) > 3)))
+    assert np.mean(np.abs(z) > 3) <= 0.02
+    assert np.abs(z).max() < 6
+    assert 0.5 < np.mean(z * z) < 1.6
+    # whole-image mean within 3 sigma
+    tot = (g.astype(np.float64).mean((0, 1)) - c.astype(np.float64).mean((0, 1)))
+    se_tot = np.sqrt(((gsq - g.astype(np.float64) ** 2).mean((0, 1)) + (csq - c.astype(np.float64) ** 2).mean((0, 1)))
+                     / (spp * W * H))
+    assert np.all(np.abs(tot) <= 3 * se_tot + 1e-7), (tot, se_tot)
+
+
+def _cie():
+    return np.array([po.wavelength_into_xyz(l) for l in range(380, 780)], np.float64)
+
+
+def test_direct_emitter_closed_form(renderer):
+    sid = renderer.upload(direct_emitter())
+    W = H = 64
+    spp = 64
+    g, gsq = renderer.render(sid, abi.render_opts(W, H, spp, 4, seed=3), want_sq=True)
+    want = expected_pixel([po.spectral_power((1, 1, 1), l) for l in range(380, 780)], _cie())
+    mean = g.reshape(-1, 3).astype(np.float64).mean(0)
+    var = (gsq.reshape(-1, 3) - g.reshape(-1, 3).astype(np.float64) ** 2).mean(0)
+    se = np.sqrt(var / (W * H * spp)) + 1e-9
+    assert np.all(np.abs(mean - want) < 4 * se), (mean, want, se)
+
+
+def test_white_furnace_closed_form(renderer):
+    a, b = 0.5, 0.5
+    s = furnace(a, b)
+    sid = renderer.upload(s)
+    W = H = 64
+    spp = 128
+    g, gsq = renderer.render(sid, abi.render_opts(W, H, spp, 6, seed=5), want_sq=True)
+    gg = float(s.background[0])
+    sky = np.array([po.spectral_power((gg, gg, gg), l) for l in range(380, 780)])
+    alb = np.array([po.spectral_power((a, a, a), l) for l in range(380, 780)])
+    want = expected_pixel(sky * alb, _cie())
+    mask = sphere_mask(W, H)
+    m = g[mask].astype(np.float64).mean(0)
+    v = (gsq[mask] - g[mask].astype(np.float64) ** 2).mean(0)
+    se = np.sqrt(v / (mask.sum() * spp))
+    assert np.all(np.abs(m - want) < 4 * se + 1e-4 * want), (m, want, se)
+
+
+def test_determinism_and_sample_offset(renderer):
+    sid = gpu_scene(renderer, "cornell")
+    o = abi.render_opts(64, 64, 8, 8, seed=9)
+    a, _ = renderer.render(sid, o)
+    b, _ = renderer.render(sid, o)
+    assert np.array_equal(a, b)
+    h1, _ = renderer.render(sid, abi.render_opts(64, 64, 4, 8, seed=9))
+    h2, _ = renderer.render(sid, abi.render_opts(64, 64, 4, 8, seed=9, sample_offset=4))
+    assert np.allclose(a, 0.5 * (h1 + h2), rtol=1e-5, atol=1e-7)
+    c, _ = renderer.render(sid, abi.render_opts(64, 64, 8, 8, seed=9, fp64_accum=1))
+    assert np.allclose(a, c, rtol=1e-5, atol=1e-7)
+
+
+def test_depth_zero(renderer):
+    sid = gpu_scene(renderer, "furnace")
+    g, _ = renderer.render(sid, abi.render_opts(32, 32, 4, 0, seed=1))
+    assert np.all(g[sphere_mask(32, 32)] == 0.0)
+
+
+def test_counters_and_counting_mode(renderer):
+    sid = gpu_scene(renderer, "spheres")
+    o = abi.render_opts(160, 90, 4, 16, seed=1, flags=abi.FLAG_COUNT | abi.FLAG_TIME_KERNELS)
+    renderer.render(sid, o)
+    c = renderer.counters()
+    assert c["samples"] == 160 * 90 * 4
+    assert c["rays"] >= c["samples"]
+    assert c["node_visits"] >= c["rays"]
+    assert c["prim_tests"] > 0
+    assert 0 < c["extend_ms"] <= c["render_ms"]
+    assert c["extend_launches"] == c["passes"] * 17
+
+
+def test_error_paths(renderer):
+    import clovers_amd
+    sid = gpu_scene(renderer, "cornell")
+    with pytest.raises(clovers_amd.ClvError) as e:
+        renderer.render(sid, abi.render_opts(0, 64, 1, 8))
+    assert e.value.status == abi.EINVAL
+    with pytest.raises(clovers_amd.ClvError) as e:
+        renderer.render(12345, abi.render_opts(8, 8, 1, 8))
+    assert e.value.status == abi.EINVAL
+    with pytest.raises(clovers_amd.ClvError) as e:
+        renderer.render(sid, abi.render_opts(8, 8, 0, 8))
+    assert e.value.status == abi.EINVAL
