@@ -69,6 +69,9 @@ struct clv_ctx {
   /* misc */
   uint32_t *counts = nullptr;
   uint64_t *stats = nullptr;
+  uint32_t *spill = nullptr;      /* traversal stack overflow, one slice per resident thread */
+  uint32_t trace_blocks = 0;      /* resident blocks of the persistent traversal */
+  int trace_variant = TRACE_SIMPLE;
   float4 *cie = nullptr;
   std::vector<hipEvent_t> events;
   clv_counters counters;
@@ -213,8 +216,12 @@ clv_status clv_create(int hip_device, clv_ctx **out) {
     delete c;
     return CLV_EDEVICE;
   }
+  c->trace_blocks = trace_grid_blocks();
+  if (const char *e = getenv("CLV_TRACE_BLOCKS")) c->trace_blocks = (uint32_t)strtoul(e, nullptr, 10);
+  if (const char *e = getenv("CLV_TRACE_KERNEL")) c->trace_variant = strcmp(e, "persistent") == 0 ? TRACE_PERSISTENT : TRACE_SIMPLE;
+  if (c->trace_blocks < 1) c->trace_blocks = 1;
   if (dalloc(&c->counts, CNT_TOTAL) != hipSuccess || dalloc(&c->stats, 4) != hipSuccess ||
-      dalloc(&c->cie, CLV_SPECTRUM) != hipSuccess) {
+      dalloc(&c->cie, CLV_SPECTRUM) != hipSuccess || dalloc(&c->spill, trace_spill_words(c->trace_blocks)) != hipSuccess) {
     clv_destroy(c);
     return CLV_ENOMEM;
   }
@@ -241,6 +248,7 @@ void clv_destroy(clv_ctx *c) {
   free_pixels(c);
   dfree(c->counts);
   dfree(c->stats);
+  dfree(c->spill);
   dfree(c->cie);
   for (auto e : c->events) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -397,8 +405,15 @@ clv_status clv_render(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, f
       uint32_t in_slot = in ? CNT_QB : CNT_QA, out_slot = out ? CNT_QB : CNT_QA;
       launch_reset_counts(c->stream, c->counts, in_slot, out_slot, c->stats);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev], c->stream));
-      launch_extend(c->stream, ds, c->pb, c->queue[in], c->counts, in_slot, sq, gp.paths, c->grid_cap,
-                    counting ? c->stats : nullptr);
+      TraceArgs ta;
+      memset(&ta, 0, sizeof(ta));
+      ta.qin = c->queue[in];
+      ta.counts = c->counts;
+      ta.in_slot = in_slot;
+      ta.sq = sq;
+      ta.spill = c->spill;
+      ta.stats = (unsigned long long *)c->stats;
+      launch_extend(c->stream, ds, c->pb, ta, c->trace_blocks, counting, c->trace_variant);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
       ev += timing ? 2 : 0;
       launch_shade(c->stream, ds, c->pb, sq, c->queue[out], c->counts, out_slot, o->max_depth, gp.paths, c->grid_cap);
@@ -464,7 +479,15 @@ clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   gp.sample_base = o->sample_offset;
   gp.seed = o->seed;
   launch_generate(c->stream, ds, c->pb, gp, c->queue[0]);
-  launch_primary(c->stream, ds, c->pb, WH, c->ids, c->ts);
+  launch_set_count(c->stream, c->counts, CNT_FETCH, 0u);
+  TraceArgs ta;
+  memset(&ta, 0, sizeof(ta));
+  ta.counts = c->counts;
+  ta.n_direct = WH;
+  ta.spill = c->spill;
+  ta.ids = c->ids;
+  ta.ts = c->ts;
+  launch_primary(c->stream, ds, c->pb, ta, c->trace_blocks, c->trace_variant);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(ids, c->ids, sizeof(uint32_t) * WH, hipMemcpyDeviceToHost, c->stream));
   if (ts) HIPCHK(c, hipMemcpyAsync(ts, c->ts, sizeof(float) * WH, hipMemcpyDeviceToHost, c->stream));

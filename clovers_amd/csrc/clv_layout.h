@@ -6,7 +6,8 @@
  *   nodes      BVH4 nodes, 128 B each, 128-B aligned: child boxes as SoA
  *              (lo/hi per axis x 4 children) + 4 child words.
  *   prims      variable-size primitive records in float4 units:
- *                sphere 2 x float4 (32 B), triangle 4 x float4 (64 B),
+ *                sphere 2 x float4 (32 B), triangle 4 x float4 (64 B: corners,
+ *                plane d, unit normal),
  *                quad 5 x float4 (80 B); word .w of float4 #0 is the header
  *              (type | (xform+1) << 2), word .w of float4 #1 the primitive
  *              index into prim_info.

@@ -220,8 +220,8 @@ void write_record(std::vector<float> &out, const FlatPrim &p) {
   if (p.type == PT_SPHERE) {        /* [c, hdr] [r, 0, 0, info] */
     float r[8] = {d[0], d[1], d[2], fh, d[3], z, z, fi};
     out.insert(out.end(), r, r + 8);
-  } else if (p.type == PT_TRI) {    /* [a, hdr] [b, info] [c, 0] [n, 0] */
-    float r[16] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], z, d[9], d[10], d[11], z};
+  } else if (p.type == PT_TRI) {    /* [a, hdr] [b, info] [c, plane d] [n, 0] */
+    float r[16] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], d[12], d[9], d[10], d[11], z};
     out.insert(out.end(), r, r + 16);
   } else {                          /* [q, hdr] [u, info] [v, d] [n, area] [w, 0] */
     float r[20] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], d[12],
@@ -739,6 +739,7 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
           q[3] = vb.x; q[4] = vb.y; q[5] = vb.z;
           q[6] = vc.x; q[7] = vc.y; q[8] = vc.z;
           q[9] = pl.n.x; q[10] = pl.n.y; q[11] = pl.n.z;
+          q[12] = pl.d;
         } else {
           fp.type = PT_QUAD;
           quad_payload(&fp, pl);

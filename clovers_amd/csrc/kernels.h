@@ -14,6 +14,7 @@ namespace clv {
 #define CNT_QA 0u       /* ray queue A size */
 #define CNT_QB 1u       /* ray queue B size */
 #define CNT_SHADE 2u    /* MC_COUNT shading queue sizes start here */
+#define CNT_FETCH 7u    /* work counter of the persistent traversal */
 #define CNT_TOTAL 8u
 
 struct DevScene {
@@ -50,6 +51,19 @@ struct ShadeQueues {
   uint32_t *counts; /* = counter array + CNT_SHADE */
 };
 
+/* arguments of the persistent traversal kernel */
+struct TraceArgs {
+  const uint32_t *qin;         /* extend mode: ray queue */
+  uint32_t *counts;            /* counter array (queue sizes, fetch counter) */
+  uint32_t in_slot;            /* extend mode: slot of the ray-queue size */
+  uint32_t n_direct;           /* primary mode: number of rays (paths 0..n-1) */
+  ShadeQueues sq;              /* extend mode: shading queues */
+  uint32_t *spill;             /* per-thread overflow of the traversal stack */
+  unsigned long long *stats;   /* counting mode: node visits, primitive tests */
+  uint32_t *ids;               /* primary mode outputs */
+  float *ts;
+};
+
 struct GenParams {
   uint32_t width, height, pixels, paths;  /* paths = pixels * samples in this pass */
   uint32_t sample_base;
@@ -71,14 +85,23 @@ struct ResolveParams {
 void launch_reset_counts(hipStream_t st, uint32_t *counts, uint32_t in_slot, uint32_t out_slot, uint64_t *stats);
 void launch_set_count(hipStream_t st, uint32_t *counts, uint32_t slot, uint32_t value);
 void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp, uint32_t *qout);
-void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const uint32_t *qin, const uint32_t *counts,
-                   uint32_t in_slot, const ShadeQueues &sq, uint32_t max_paths, uint32_t grid_cap, uint64_t *stats);
+/* persistent traversal: grid = resident blocks of the device; spill buffer of
+ * trace_spill_words(blocks) words */
+uint32_t trace_grid_blocks();
+size_t trace_spill_words(uint32_t blocks);
+/* traversal kernel variants: one ray per thread over a static grid-stride
+ * loop, or persistent waves refilling lanes from an atomic work counter */
+#define TRACE_SIMPLE 0
+#define TRACE_PERSISTENT 1
+void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
+                   bool counting, int variant);
+void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
+                    int variant);
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
                   uint32_t *counts, uint32_t out_slot, uint32_t max_depth, uint32_t max_paths, uint32_t grid_cap);
 void launch_resolve(hipStream_t st, const PathBuf &pb, const ResolveParams &rp, const float4 *cie);
 void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float *out_sq, uint32_t spp);
 void launch_zero_accum(hipStream_t st, const ResolveParams &rp);
-void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, uint32_t n, uint32_t *ids, float *ts);
 
 }  // namespace clv
 #endif

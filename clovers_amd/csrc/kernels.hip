@@ -141,12 +141,18 @@ __device__ __forceinline__ WRay wray(V d) {
   return w;
 }
 
-/* Watertight ray/triangle test with the reference's acceptance rules kept:
- * |n.d| < 1e-4 is rejected and the t bounds are inclusive (triangle.rs:129-139). */
+/* Triangle test: the watertight inside test (no gaps or double hits along
+ * shared edges) decides the hit; the reference's rules decide the rest --
+ * |n.d| < 1e-4 is rejected, and the distance is the reference's plane
+ * distance t = (-d - n.o) / (n.d) with inclusive bounds (triangle.rs:129-139),
+ * so distances agree with the reference's to the last bit. */
 __device__ __forceinline__ bool tri_t(const float4 *rec, V o, V d, const WRay &wr, float tmin, float tmax, float &t) {
   float4 f0 = rec[0], f1 = rec[1], f2 = rec[2], f3 = rec[3];
   V n = xyz(f3);
-  if (fabsf(dot(n, d)) < EPS_RECT) return false;
+  float denom = dot(n, d);
+  if (fabsf(denom) < EPS_RECT) return false;
+  float tt = (-f2.w - dot(n, o)) / denom;
+  if (!(tt >= tmin && tt <= tmax)) return false; /* also rejects NaN */
   V A = xyz(f0) - o, B = xyz(f1) - o, C = xyz(f2) - o;
   float Akx = comp(A, wr.kx), Aky = comp(A, wr.ky), Akz = comp(A, wr.kz);
   float Bkx = comp(B, wr.kx), Bky = comp(B, wr.ky), Bkz = comp(B, wr.kz);
@@ -158,11 +164,7 @@ __device__ __forceinline__ bool tri_t(const float4 *rec, V o, V d, const WRay &w
   float Vv = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
   if ((U < 0.0f || Vv < 0.0f || W < 0.0f) && (U > 0.0f || Vv > 0.0f || W > 0.0f)) return false;
-  float det = U + Vv + W;
-  if (det == 0.0f) return false;
-  float T = U * (wr.sz * Akz) + Vv * (wr.sz * Bkz) + W * (wr.sz * Ckz);
-  float tt = T / det;
-  if (!(tt >= tmin && tt <= tmax)) return false;
+  if (U + Vv + W == 0.0f) return false;
   t = tt;
   return true;
 }
@@ -205,100 +207,119 @@ __device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec,
 }
 
 /* ------------------------------------------------------ traversal --- */
-#define STACK_LDS 24
-#define STACK_SPILL 48
+/* Per-lane traversal stack: the top STACK_LDS entries live in LDS
+ * (interleaved so that a wave's lanes hit distinct banks), deeper entries in
+ * a per-thread slice of a global buffer.  The host refuses BVHs whose depth
+ * could need more than STACK_TOTAL entries (3 pending children per level). */
+#define STACK_LDS 16
+#define STACK_TOTAL 72
+#define STACK_GLOBAL (STACK_TOTAL - STACK_LDS)
+#define TRACE_BLOCK 256
 
-/* Closest hit over the 4-wide BVH.  Returns the record offset of the hit
- * primitive (or MISS_WORD) and its distance in tbest.  Every primitive keeps
- * its own acceptance rule relative to the current closest distance, as in the
- * reference (inclusive for quads/triangles, strict for spheres). */
-template <bool COUNT>
-__device__ uint32_t closest_hit(const DevScene &sc, V o, V d, float &tbest, uint32_t *lds, uint32_t lane_slot,
-                                uint32_t stride, uint32_t *nodes_visited, uint32_t *prims_tested) {
-  const float tmin = EPS_ACNE;
-  float tb = FLT_MAX;
-  uint32_t best = MISS_WORD;
-  V dd = mk(fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x, fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y,
-            fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z);
-  V inv = mk(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
-  V oi = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
-  WRay wr = wray(d);
-  uint32_t spill[STACK_SPILL];
-  int sp = 0;
-  uint32_t word = 0u; /* root node */
-  uint32_t nv = 0, np = 0;
-  const bool any_xform = sc.has_xforms != 0;
-  for (;;) {
-    if (!(word & CHILD_LEAF)) {
-      const Bvh4Node *nd = sc.nodes + word;
-      const float4 *nf = reinterpret_cast<const float4 *>(nd);
-      float4 lx = nf[0], hx = nf[1], ly = nf[2], hy = nf[3], lz = nf[4], hz = nf[5];
-      uint4 ch = reinterpret_cast<const uint4 *>(nd)[6];
-      if (COUNT) nv++;
-      float tn[4];
-      uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
-      float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-      float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-      float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-      const float tfar_lim = tb * 1.00000024f;
+/* state of one lane's ray */
+struct Lane {
+  V o, d, inv, oi;
+  WRay wr;
+  float tb;          /* closest accepted distance so far (the reference starts at f32::MAX, trace.rs:44) */
+  uint32_t best;     /* record offset of the closest hit, MISS_WORD if none */
+  uint32_t word;     /* node / leaf word being processed, CHILD_EMPTY when the traversal is over */
+  uint32_t path;
+  int sp;
+  bool live;
+};
+
+__device__ __forceinline__ void lane_start(Lane &L, const PathBuf &pb, uint32_t path) {
+  float4 o4 = pb.ray_o[path], d4 = pb.ray_d[path];
+  L.o = xyz(o4);
+  L.d = xyz(d4);
+  V dd = mk(fabsf(L.d.x) < 1e-20f ? copysignf(1e-20f, L.d.x) : L.d.x,
+            fabsf(L.d.y) < 1e-20f ? copysignf(1e-20f, L.d.y) : L.d.y,
+            fabsf(L.d.z) < 1e-20f ? copysignf(1e-20f, L.d.z) : L.d.z);
+  L.inv = mk(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  L.oi = mk(L.o.x * L.inv.x, L.o.y * L.inv.y, L.o.z * L.inv.z);
+  L.wr = wray(L.d);
+  L.tb = FLT_MAX;
+  L.best = MISS_WORD;
+  /* a ray with a NaN component misses at the root, as in the reference, where
+   * both child entry distances come out None (aabb.rs:135-155,
+   * bvh/hitable_trait.rs:43-46); without this the NaN-blind slab test below
+   * would accept every box and walk the whole tree */
+  const float chk = L.o.x + L.o.y + L.o.z + L.d.x + L.d.y + L.d.z;
+  L.word = (chk == chk) ? 0u : CHILD_EMPTY; /* root node */
+  L.sp = 0;
+  L.path = path;
+  L.live = true;
+}
+
+__device__ __forceinline__ void lane_push(Lane &L, uint32_t w, uint32_t *lst, uint32_t *gst) {
+  if (L.sp < STACK_LDS) lst[(uint32_t)L.sp * TRACE_BLOCK] = w;
+  else if (L.sp < STACK_TOTAL) gst[L.sp - STACK_LDS] = w;
+  else return; /* unreachable: the host bounds the BVH depth */
+  L.sp++;
+}
+
+__device__ __forceinline__ void lane_pop(Lane &L, const uint32_t *lst, const uint32_t *gst) {
+  if (L.sp == 0) { L.word = CHILD_EMPTY; return; }
+  L.sp--;
+  L.word = L.sp < STACK_LDS ? lst[(uint32_t)L.sp * TRACE_BLOCK] : gst[L.sp - STACK_LDS];
+}
+
+/* one interior node: test the four child boxes, continue with the nearest
+ * hit child, push the others far-to-near */
+__device__ __forceinline__ void lane_node(Lane &L, const DevScene &sc, uint32_t *lst, uint32_t *gst) {
+  const Bvh4Node *nd = sc.nodes + L.word;
+  const float4 *nf = reinterpret_cast<const float4 *>(nd);
+  float4 lx = nf[0], hx = nf[1], ly = nf[2], hy = nf[3], lz = nf[4], hz = nf[5];
+  uint4 ch = reinterpret_cast<const uint4 *>(nd)[6];
+  float tn[4];
+  uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
+  float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+  float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+  float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+  const float tfar_lim = L.tb * 1.00000024f;
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        float x0 = lxa[k] * inv.x - oi.x, x1 = hxa[k] * inv.x - oi.x;
-        float y0 = lya[k] * inv.y - oi.y, y1 = hya[k] * inv.y - oi.y;
-        float z0 = lza[k] * inv.z - oi.z, z1 = hza[k] * inv.z - oi.z;
-        float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-        float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
-        bool hit = (tnear <= tfar) && (cw[k] != CHILD_EMPTY);
-        tn[k] = hit ? tnear : FLT_MAX;
-      }
-      /* sort the four (t, child) pairs ascending: 5-comparator network */
+  for (int k = 0; k < 4; k++) {
+    float x0 = fmaf(lxa[k], L.inv.x, -L.oi.x), x1 = fmaf(hxa[k], L.inv.x, -L.oi.x);
+    float y0 = fmaf(lya[k], L.inv.y, -L.oi.y), y1 = fmaf(hya[k], L.inv.y, -L.oi.y);
+    float z0 = fmaf(lza[k], L.inv.z, -L.oi.z), z1 = fmaf(hza[k], L.inv.z, -L.oi.z);
+    float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), EPS_ACNE));
+    float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
+    bool hit = (tnear <= tfar) && (cw[k] != CHILD_EMPTY);
+    tn[k] = hit ? tnear : FLT_MAX;
+  }
 #define CSWAP(a, b)                                                       \
   if (tn[b] < tn[a]) {                                                    \
     float tt_ = tn[a]; tn[a] = tn[b]; tn[b] = tt_;                        \
     uint32_t cc_ = cw[a]; cw[a] = cw[b]; cw[b] = cc_;                     \
   }
-      CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+  CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
 #undef CSWAP
-      /* push far children (hits only) so the nearest is popped first */
 #pragma unroll
-      for (int k = 3; k >= 1; k--) {
-        if (tn[k] < FLT_MAX && sp < STACK_LDS + STACK_SPILL) { /* the host bounds the depth so this always holds */
-          if (sp < STACK_LDS) lds[(uint32_t)sp * stride + lane_slot] = cw[k];
-          else spill[sp - STACK_LDS] = cw[k];
-          sp++;
-        }
-      }
-      if (tn[0] < FLT_MAX) {
-        word = cw[0];
-        continue;
-      }
-    } else {
-      uint32_t cnt = ((word >> 28) & 7u) + 1u;
-      uint32_t off = word & LEAF_OFFSET_MASK;
-      for (uint32_t k = 0; k < cnt; k++) {
-        const float4 *rec = sc.prims + off;
-        uint32_t hdr = as_u(rec[0].w);
-        float t;
-        bool h = any_xform ? test_prim<true>(sc, rec, hdr, o, d, wr, tmin, tb, t)
-                           : test_prim<false>(sc, rec, hdr, o, d, wr, tmin, tb, t);
-        if (COUNT) np++;
-        if (h) {
-          tb = t;
-          best = off;
-        }
-        off += prim_units(hdr & 3u);
-      }
+  for (int k = 3; k >= 1; k--)
+    if (tn[k] < FLT_MAX) lane_push(L, cw[k], lst, gst);
+  if (tn[0] < FLT_MAX) L.word = cw[0];
+  else lane_pop(L, lst, gst);
+}
+
+/* one leaf: test its primitives; every primitive keeps its own acceptance
+ * rule against the current closest distance, as in the reference (inclusive
+ * for quads and triangles, strict for spheres) */
+__device__ __forceinline__ uint32_t lane_leaf(Lane &L, const DevScene &sc, bool any_xform) {
+  uint32_t cnt = ((L.word >> 28) & 7u) + 1u;
+  uint32_t off = L.word & LEAF_OFFSET_MASK;
+  for (uint32_t k = 0; k < cnt; k++) {
+    const float4 *rec = sc.prims + off;
+    uint32_t hdr = as_u(rec[0].w);
+    float t;
+    bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t)
+                       : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t);
+    if (h) {
+      L.tb = t;
+      L.best = off;
     }
-    if (sp == 0) break;
-    sp--;
-    word = sp < STACK_LDS ? lds[(uint32_t)sp * stride + lane_slot] : spill[sp - STACK_LDS];
+    off += prim_units(hdr & 3u);
   }
-  if (COUNT) {
-    *nodes_visited += nv;
-    *prims_tested += np;
-  }
-  tbest = tb;
-  return best;
+  return cnt;
 }
 
 /* ------------------------------------------------------------ kernels --- */
@@ -311,6 +332,7 @@ __global__ void k_reset_counts(uint32_t *counts, uint32_t in_slot, uint32_t out_
   __syncthreads();
   if (i < MC_COUNT) counts[CNT_SHADE + i] = 0u;
   if (i == 0) counts[out_slot] = 0u;
+  if (i == 1) counts[CNT_FETCH] = 0u;
 }
 
 __global__ void k_set_count(uint32_t *counts, uint32_t slot, uint32_t value) {
@@ -357,34 +379,147 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
   }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_extend(DevScene sc, PathBuf pb, const uint32_t *qin, const uint32_t *counts,
-                                                 uint32_t in_slot, ShadeQueues sq, uint32_t *stats) {
-  __shared__ uint32_t stack[STACK_LDS * 256];
-  const uint32_t n = counts[in_slot];
+/* Closest-hit kernel, one ray per thread: the block's threads take rays
+ * base + threadIdx.x in a grid-stride loop; each thread runs its ray to
+ * completion (node or leaf per step).  Results as in k_trace below. */
+template <bool COUNT, int MODE>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathBuf pb, TraceArgs ta) {
+  __shared__ uint32_t lds_stack[STACK_LDS * TRACE_BLOCK];
+  const uint32_t n = MODE == 0 ? ta.counts[ta.in_slot] : ta.n_direct;
+  uint32_t *lst = lds_stack + threadIdx.x;
+  uint32_t *gst = ta.spill + (size_t)(blockIdx.x * TRACE_BLOCK + threadIdx.x) * STACK_GLOBAL;
+  const bool any_xform = sc.has_xforms != 0;
   uint32_t nvisit = 0, ntest = 0;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    uint32_t i = base + threadIdx.x;
-    bool active = i < n;
-    uint32_t path = 0, cls = MC_MISS;
+  for (uint32_t base = blockIdx.x * TRACE_BLOCK; base < n; base += gridDim.x * TRACE_BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    const bool active = i < n;
+    Lane L;
+    L.live = false;
+    L.best = MISS_WORD;
+    L.path = 0;
     if (active) {
-      path = qin[i];
-      float4 o4 = pb.ray_o[path], d4 = pb.ray_d[path];
-      float t;
-      uint32_t rec = closest_hit<COUNT>(sc, xyz(o4), xyz(d4), t, stack, threadIdx.x, 256u, &nvisit, &ntest);
-      pb.hit_t[path] = t;
-      pb.hit_p[path] = rec;
-      if (rec != MISS_WORD) {
-        uint32_t info = as_u(sc.prims[rec + 1u].w);
-        cls = sc.mats[sc.info[info].material].mclass;
+      lane_start(L, pb, MODE == 0 ? ta.qin[i] : i);
+      while (L.word != CHILD_EMPTY) {
+        if (!(L.word & CHILD_LEAF)) {
+          lane_node(L, sc, lst, gst);
+          if (COUNT) nvisit++;
+        } else {
+          uint32_t c = lane_leaf(L, sc, any_xform);
+          if (COUNT) ntest += c;
+          lane_pop(L, lst, gst);
+        }
       }
     }
+    if (MODE == 0) {
+      uint32_t cls = MC_MISS;
+      if (active) {
+        pb.hit_t[L.path] = L.tb;
+        pb.hit_p[L.path] = L.best;
+        if (L.best != MISS_WORD) cls = sc.mats[sc.info[as_u(sc.prims[L.best + 1u].w)].material].mclass;
+      }
 #pragma unroll
-    for (uint32_t c = 0; c < MC_COUNT; c++) wave_append(active && cls == c, path, sq.q[c], sq.counts + c);
+      for (uint32_t c = 0; c < MC_COUNT; c++) wave_append(active && cls == c, L.path, ta.sq.q[c], ta.sq.counts + c);
+    } else if (active) {
+      if (L.best == MISS_WORD) {
+        ta.ids[L.path] = 0xFFFFFFFFu;
+        ta.ts[L.path] = INFINITY;
+      } else {
+        ta.ids[L.path] = sc.info[as_u(sc.prims[L.best + 1u].w)].report_id;
+        ta.ts[L.path] = L.tb;
+      }
+    }
   }
   if (COUNT) {
-    atomicAdd((unsigned long long *)&stats[0], (unsigned long long)nvisit);
-    atomicAdd((unsigned long long *)&stats[2], (unsigned long long)ntest);
+    atomicAdd(ta.stats + 0, (unsigned long long)nvisit);
+    atomicAdd(ta.stats + 1, (unsigned long long)ntest);
+  }
+}
+
+/* Persistent closest-hit kernel.  Each wave keeps 64 rays in flight; a lane
+ * whose ray is finished takes the next one from the queue (one atomic per
+ * wave per refill), so waves stay full through the tail of a bounce.  Each
+ * iteration descends interior nodes until every lane holds a leaf
+ * (while-while), then tests one leaf per lane.  Finished rays are written out
+ * and, in the extend mode, appended to the shading queue of their material
+ * class with wave-aggregated atomics.
+ *   MODE_EXTEND:  rays are qin[0 .. counts[in_slot]) of the path state
+ *   MODE_PRIMARY: rays are paths 0 .. n_direct, results go to ids/ts */
+#define MODE_EXTEND 0
+#define MODE_PRIMARY 1
+
+template <bool COUNT, int MODE>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, PathBuf pb, TraceArgs ta) {
+  __shared__ uint32_t lds_stack[STACK_LDS * TRACE_BLOCK];
+  const uint32_t n = MODE == MODE_EXTEND ? ta.counts[ta.in_slot] : ta.n_direct;
+  const uint32_t lane = __lane_id();
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  uint32_t *lst = lds_stack + threadIdx.x;
+  uint32_t *gst = ta.spill + (size_t)(blockIdx.x * TRACE_BLOCK + threadIdx.x) * STACK_GLOBAL;
+  const bool any_xform = sc.has_xforms != 0;
+  uint32_t nvisit = 0, ntest = 0;
+  bool exhausted = false;
+  Lane L;
+  L.live = false;
+  L.word = CHILD_EMPTY;
+  L.path = 0;
+  L.sp = 0;
+  L.tb = FLT_MAX;
+  L.best = MISS_WORD;
+  for (;;) {
+    if (!exhausted) {
+      uint64_t idle = __ballot(!L.live);
+      if (idle) {
+        uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(ta.counts + CNT_FETCH, (uint32_t)__popcll(idle));
+        base = __shfl(base, (int)leader);
+        if (base >= n) exhausted = true;
+        if (!L.live) {
+          uint32_t idx = base + (uint32_t)__popcll(idle & lt_mask);
+          if (idx < n) lane_start(L, pb, MODE == MODE_EXTEND ? ta.qin[idx] : idx);
+        }
+      }
+    }
+    if (!__any(L.live)) {
+      if (exhausted) break;
+      continue;
+    }
+    /* descend interior nodes until every live lane holds a leaf or is done */
+    while (__any(L.live && !(L.word & CHILD_LEAF))) {
+      if (L.live && !(L.word & CHILD_LEAF)) {
+        lane_node(L, sc, lst, gst);
+        if (COUNT) nvisit++;
+      }
+    }
+    if (L.live && L.word != CHILD_EMPTY) {
+      uint32_t c = lane_leaf(L, sc, any_xform);
+      if (COUNT) ntest += c;
+      lane_pop(L, lst, gst);
+    }
+    const bool fin = L.live && L.word == CHILD_EMPTY;
+    if (MODE == MODE_EXTEND) {
+      uint32_t cls = MC_MISS;
+      if (fin) {
+        pb.hit_t[L.path] = L.tb;
+        pb.hit_p[L.path] = L.best;
+        if (L.best != MISS_WORD) cls = sc.mats[sc.info[as_u(sc.prims[L.best + 1u].w)].material].mclass;
+      }
+#pragma unroll
+      for (uint32_t c = 0; c < MC_COUNT; c++) wave_append(fin && cls == c, L.path, ta.sq.q[c], ta.sq.counts + c);
+    } else if (fin) {
+      if (L.best == MISS_WORD) {
+        ta.ids[L.path] = 0xFFFFFFFFu;
+        ta.ts[L.path] = INFINITY;
+      } else {
+        ta.ids[L.path] = sc.info[as_u(sc.prims[L.best + 1u].w)].report_id;
+        ta.ts[L.path] = L.tb;
+      }
+    }
+    if (fin) L.live = false;
+  }
+  if (COUNT) {
+    atomicAdd(ta.stats + 0, (unsigned long long)nvisit);
+    atomicAdd(ta.stats + 1, (unsigned long long)ntest);
   }
 }
 
@@ -801,25 +936,6 @@ __global__ void k_zero_accum(ResolveParams rp) {
   }
 }
 
-/* primary-ray probe: closest hit of the camera rays written by k_generate */
-__global__ __launch_bounds__(256) void k_primary(DevScene sc, PathBuf pb, uint32_t n, uint32_t *ids, float *ts) {
-  __shared__ uint32_t stack[STACK_LDS * 256];
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    uint32_t i = base + threadIdx.x;
-    if (i >= n) continue;
-    float t;
-    uint32_t rec = closest_hit<false>(sc, xyz(pb.ray_o[i]), xyz(pb.ray_d[i]), t, stack, threadIdx.x, 256u, nullptr,
-                                      nullptr);
-    if (rec == MISS_WORD) {
-      ids[i] = 0xFFFFFFFFu;
-      ts[i] = INFINITY;
-    } else {
-      ids[i] = sc.info[as_u(sc.prims[rec + 1u].w)].report_id;
-      ts[i] = t;
-    }
-  }
-}
-
 /* ---------------------------------------------------------- launchers --- */
 static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
   uint32_t g = (n + block - 1) / block;
@@ -836,13 +952,44 @@ void launch_set_count(hipStream_t st, uint32_t *counts, uint32_t slot, uint32_t 
 void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp, uint32_t *qout) {
   hipLaunchKernelGGL(k_generate, dim3(grid_for(gp.paths, 256, 65535)), dim3(256), 0, st, sc, pb, gp, qout);
 }
-void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const uint32_t *qin, const uint32_t *counts,
-                   uint32_t in_slot, const ShadeQueues &sq, uint32_t max_paths, uint32_t grid_cap, uint64_t *stats) {
-  uint32_t g = grid_for(max_paths, 256, grid_cap);
-  if (stats)
-    hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(256), 0, st, sc, pb, qin, counts, in_slot, sq, (uint32_t *)stats);
+static uint32_t g_trace_blocks = 0;
+
+uint32_t trace_grid_blocks() {
+  if (g_trace_blocks) return g_trace_blocks;
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_trace<false, MODE_EXTEND>),
+                                                      TRACE_BLOCK, 0);
+  if (cus < 1) cus = 1;
+  if (per_cu < 1) per_cu = 1;
+  g_trace_blocks = (uint32_t)(cus * per_cu);
+  return g_trace_blocks;
+}
+
+size_t trace_spill_words(uint32_t blocks) { return (size_t)blocks * TRACE_BLOCK * STACK_GLOBAL; }
+
+void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
+                   bool counting, int variant) {
+  if (variant == TRACE_PERSISTENT) {
+    if (counting)
+      hipLaunchKernelGGL((k_trace<true, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+    else
+      hipLaunchKernelGGL((k_trace<false, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+  } else {
+    if (counting)
+      hipLaunchKernelGGL((k_trace_simple<true, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+    else
+      hipLaunchKernelGGL((k_trace_simple<false, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+  }
+}
+
+void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
+                    int variant) {
+  if (variant == TRACE_PERSISTENT)
+    hipLaunchKernelGGL((k_trace<false, MODE_PRIMARY>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
   else
-    hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(256), 0, st, sc, pb, qin, counts, in_slot, sq, nullptr);
+    hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
 }
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
                   uint32_t *counts, uint32_t out_slot, uint32_t max_depth, uint32_t max_paths, uint32_t grid_cap) {
@@ -857,9 +1004,6 @@ void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float 
 }
 void launch_zero_accum(hipStream_t st, const ResolveParams &rp) {
   hipLaunchKernelGGL(k_zero_accum, dim3(grid_for(rp.pixels * 3u, 256, 65535)), dim3(256), 0, st, rp);
-}
-void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, uint32_t n, uint32_t *ids, float *ts) {
-  hipLaunchKernelGGL(k_primary, dim3(grid_for(n, 256, 65535)), dim3(256), 0, st, sc, pb, n, ids, ts);
 }
 
 }  // namespace clv
