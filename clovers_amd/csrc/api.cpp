@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <new>
@@ -43,6 +44,7 @@ struct SceneRes {
   std::vector<DevBuf> bufs;
   clv_camera_init camera;
   uint32_t n_nodes = 0, n_prims = 0, prim_bytes = 0, depth = 0;
+  uint32_t prim_units = 0;   /* float4 units of primitive records */
   double build_ms = 0.0;
 };
 
@@ -69,9 +71,12 @@ struct clv_ctx {
   /* misc */
   uint32_t *counts = nullptr;
   uint64_t *stats = nullptr;
-  uint32_t *spill = nullptr;      /* traversal stack overflow, one slice per resident thread */
+  uint32_t *spill = nullptr;      /* traversal stack overflow, one slice per launched thread */
+  size_t spill_words = 0;         /* size of the spill buffer */
+  size_t lds_block_bytes = 64512; /* dynamic LDS budget of one block of the LDS-cached traversal (64 KB less static use) */
   uint32_t trace_blocks = 0;      /* resident blocks of the persistent traversal */
-  int trace_variant = TRACE_SIMPLE;
+  int trace_variant = TRACE_LDS;
+  bool full_grid = false;         /* one thread per possible ray instead of the resident grid */
   float4 *cie = nullptr;
   std::vector<hipEvent_t> events;
   clv_counters counters;
@@ -113,6 +118,38 @@ void free_paths(clv_ctx *c) {
   c->queue[0] = c->queue[1] = nullptr;
   for (uint32_t k = 0; k < MC_COUNT; k++) c->shadeq[k] = nullptr;
   c->cap_paths = 0;
+}
+
+clv_status ensure_spill(clv_ctx *c, size_t words) {
+  if (words <= c->spill_words) return CLV_OK;
+  dfree(c->spill);
+  c->spill = nullptr;
+  c->spill_words = 0;
+  HIPCHK(c, dalloc(&c->spill, words));
+  c->spill_words = words;
+  return CLV_OK;
+}
+
+/* traversal launch geometry of a scene for the configured variant */
+struct TraceGeom {
+  uint32_t blocks = 0;
+  size_t lds_bytes = 0;
+  uint32_t lds_nodes = 0, lds_units = 0;
+};
+
+clv_status trace_geometry(clv_ctx *c, const SceneRes *s, uint32_t full_grid_paths, TraceGeom *g) {
+  if (c->trace_variant == TRACE_LDS) {
+    size_t budget = c->lds_block_bytes > lds_trace_stack_bytes() ? c->lds_block_bytes - lds_trace_stack_bytes() : 0;
+    g->lds_nodes = (uint32_t)std::min<size_t>(s->n_nodes, budget / sizeof(Bvh4Node));
+    budget -= (size_t)g->lds_nodes * sizeof(Bvh4Node);
+    g->lds_units = (uint32_t)std::min<size_t>(s->prim_units, budget / 16u);
+    g->lds_bytes = lds_trace_bytes(g->lds_nodes, g->lds_units);
+    g->blocks = lds_trace_grid_blocks(g->lds_bytes);
+    return ensure_spill(c, lds_trace_spill_words(g->blocks));
+  }
+  g->blocks = c->trace_blocks;
+  if (full_grid_paths && c->full_grid && c->trace_variant != TRACE_PERSISTENT) g->blocks = (full_grid_paths + 255u) / 256u;
+  return ensure_spill(c, trace_spill_words(g->blocks));
 }
 
 clv_status ensure_paths(clv_ctx *c, uint32_t n) {
@@ -218,10 +255,16 @@ clv_status clv_create(int hip_device, clv_ctx **out) {
   }
   c->trace_blocks = trace_grid_blocks();
   if (const char *e = getenv("CLV_TRACE_BLOCKS")) c->trace_blocks = (uint32_t)strtoul(e, nullptr, 10);
-  if (const char *e = getenv("CLV_TRACE_KERNEL")) c->trace_variant = strcmp(e, "persistent") == 0 ? TRACE_PERSISTENT : TRACE_SIMPLE;
+  if (const char *e = getenv("CLV_TRACE_KERNEL"))
+    c->trace_variant = strcmp(e, "persistent") == 0 ? TRACE_PERSISTENT
+                       : strcmp(e, "ww") == 0       ? TRACE_WW
+                       : strcmp(e, "simple") == 0   ? TRACE_SIMPLE
+                                                    : TRACE_LDS;
+  if (const char *e = getenv("CLV_TRACE_GRID")) c->full_grid = strcmp(e, "full") == 0;
   if (c->trace_blocks < 1) c->trace_blocks = 1;
+  if (const char *e = getenv("CLV_LDS_BYTES")) c->lds_block_bytes = (size_t)strtoul(e, nullptr, 10);
   if (dalloc(&c->counts, CNT_TOTAL) != hipSuccess || dalloc(&c->stats, 4) != hipSuccess ||
-      dalloc(&c->cie, CLV_SPECTRUM) != hipSuccess || dalloc(&c->spill, trace_spill_words(c->trace_blocks)) != hipSuccess) {
+      dalloc(&c->cie, CLV_SPECTRUM) != hipSuccess) {
     clv_destroy(c);
     return CLV_ENOMEM;
   }
@@ -298,6 +341,7 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
   s->base.has_xforms = hs.xforms.empty() ? 0u : 1u;
   s->camera = hs.camera;
   s->n_nodes = (uint32_t)hs.nodes.size();
+  s->prim_units = (uint32_t)(hs.prims.size() / 4);
   s->n_prims = hs.n_prims;
   s->prim_bytes = hs.n_prims ? (uint32_t)(hs.prims.size() * 4 / hs.n_prims) : 0;
   s->depth = hs.bvh_depth;
@@ -362,6 +406,8 @@ clv_status clv_render(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, f
   if (k > o->samples) k = o->samples;
   if ((st = ensure_paths(c, WH * k)) != CLV_OK) return st;
   if ((st = ensure_pixels(c, WH)) != CLV_OK) return st;
+  TraceGeom geom;
+  if ((st = trace_geometry(c, s, WH * k, &geom)) != CLV_OK) return st;
   const DevScene ds = scene_for(s, o);
   const bool timing = (o->flags & CLV_FLAG_TIME_KERNELS) != 0;
   const bool counting = (o->flags & CLV_FLAG_COUNT) != 0;
@@ -413,7 +459,12 @@ clv_status clv_render(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, f
       ta.sq = sq;
       ta.spill = c->spill;
       ta.stats = (unsigned long long *)c->stats;
-      launch_extend(c->stream, ds, c->pb, ta, c->trace_blocks, counting, c->trace_variant);
+      ta.lds_nodes = geom.lds_nodes;
+      ta.lds_units = geom.lds_units;
+      if (c->trace_variant == TRACE_LDS)
+        launch_trace_lds(c->stream, ds, c->pb, ta, geom.blocks, geom.lds_bytes, counting, 0);
+      else
+        launch_extend(c->stream, ds, c->pb, ta, geom.blocks, counting, c->trace_variant);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
       ev += timing ? 2 : 0;
       launch_shade(c->stream, ds, c->pb, sq, c->queue[out], c->counts, out_slot, o->max_depth, gp.paths, c->grid_cap);
@@ -487,7 +538,14 @@ clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   ta.spill = c->spill;
   ta.ids = c->ids;
   ta.ts = c->ts;
-  launch_primary(c->stream, ds, c->pb, ta, c->trace_blocks, c->trace_variant);
+  TraceGeom geom;
+  if ((st = trace_geometry(c, it->second, 0, &geom)) != CLV_OK) return st;
+  ta.lds_nodes = geom.lds_nodes;
+  ta.lds_units = geom.lds_units;
+  if (c->trace_variant == TRACE_LDS)
+    launch_trace_lds(c->stream, ds, c->pb, ta, geom.blocks, geom.lds_bytes, false, 1);
+  else
+    launch_primary(c->stream, ds, c->pb, ta, geom.blocks, c->trace_variant);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(ids, c->ids, sizeof(uint32_t) * WH, hipMemcpyDeviceToHost, c->stream));
   if (ts) HIPCHK(c, hipMemcpyAsync(ts, c->ts, sizeof(float) * WH, hipMemcpyDeviceToHost, c->stream));

@@ -416,6 +416,34 @@ struct Collapser {
   }
 };
 
+/* Renumber wide nodes breadth-first (root stays 0), so that any prefix of the
+ * node array is the top of the tree: the traversal caches a prefix in LDS. */
+void renumber_bfs(std::vector<Bvh4Node> &nodes) {
+  if (nodes.size() <= 1) return;
+  std::vector<uint32_t> order;
+  order.reserve(nodes.size());
+  order.push_back(0);
+  for (size_t h = 0; h < order.size(); h++) {
+    const Bvh4Node &n = nodes[order[h]];
+    for (int k = 0; k < 4; k++) {
+      uint32_t c = n.child[k];
+      if (c != CHILD_EMPTY && !(c & CHILD_LEAF)) order.push_back(c);
+    }
+  }
+  std::vector<uint32_t> newidx(nodes.size(), 0);
+  for (size_t i = 0; i < order.size(); i++) newidx[order[i]] = (uint32_t)i;
+  std::vector<Bvh4Node> out(order.size());
+  for (size_t i = 0; i < order.size(); i++) {
+    Bvh4Node n = nodes[order[i]];
+    for (int k = 0; k < 4; k++) {
+      uint32_t c = n.child[k];
+      if (c != CHILD_EMPTY && !(c & CHILD_LEAF)) n.child[k] = newidx[c];
+    }
+    out[i] = n;
+  }
+  nodes.swap(out);
+}
+
 /* ----------------------------------------- light tree (reference rules) --- */
 struct LItem {
   Box box;          /* the reference's AABB of the hitable */
@@ -836,6 +864,7 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
   col.out.reserve(bb.nodes.size() / 2 + 1);
   col.records.reserve(prims.size() * 16);
   col.collapse(root, 0);
+  renumber_bfs(col.out);
   hs->nodes.swap(col.out);
   hs->prims.swap(col.records);
   hs->bvh_depth = col.depth;

@@ -62,6 +62,8 @@ struct TraceArgs {
   unsigned long long *stats;   /* counting mode: node visits, primitive tests */
   uint32_t *ids;               /* primary mode outputs */
   float *ts;
+  uint32_t lds_nodes;          /* LDS variant: nodes [0, lds_nodes) are cached in LDS */
+  uint32_t lds_units;          /* LDS variant: primitive float4 units [0, lds_units) are cached */
 };
 
 struct GenParams {
@@ -93,6 +95,17 @@ size_t trace_spill_words(uint32_t blocks);
  * loop, or persistent waves refilling lanes from an atomic work counter */
 #define TRACE_SIMPLE 0
 #define TRACE_PERSISTENT 1
+#define TRACE_WW 2
+#define TRACE_LDS 3
+/* LDS-cached traversal: dynamic LDS bytes for a cached prefix of `nodes` wide
+ * nodes and `units` primitive float4, resident blocks for that size, and the
+ * spill words its grid needs; mode 0 = extend, 1 = primary */
+size_t lds_trace_bytes(uint32_t nodes, uint32_t units);
+uint32_t lds_trace_stack_bytes();
+size_t lds_trace_spill_words(uint32_t blocks);
+uint32_t lds_trace_grid_blocks(size_t lds_bytes);
+void launch_trace_lds(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
+                      size_t lds_bytes, bool counting, int mode);
 void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                    bool counting, int variant);
 void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,

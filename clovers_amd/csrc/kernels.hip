@@ -88,6 +88,41 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *
   if (pred) q[base + (uint32_t)__popcll(below)] = item;
 }
 
+/* Block-aggregated append of `item` to queue `cls` (cls >= NC: no append).
+ * Each wave counts its lanes per queue with a ballot and writes the counts to
+ * LDS; NC threads turn them into per-wave offsets and do one global atomic
+ * per queue for the whole block; then every lane stores its item.  Every
+ * thread of the block must call this (it synchronises the block).  `lds`
+ * holds NC * (BLOCK/64) + NC words. */
+template <int NC, int BLOCK>
+__device__ __forceinline__ void block_append(uint32_t cls, uint32_t item, uint32_t *const *queues, uint32_t *counters,
+                                             uint32_t *lds) {
+  constexpr int WAVES = BLOCK / 64;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t rank = 0;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    uint64_t m = __ballot(cls == (uint32_t)c);
+    if (lane == 0) lds[c * WAVES + wave] = (uint32_t)__popcll(m);
+    if (cls == (uint32_t)c) rank = (uint32_t)__popcll(m & lt);
+  }
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)NC) {
+    const uint32_t c = threadIdx.x;
+    uint32_t sum = 0;
+    for (int w = 0; w < WAVES; w++) {
+      uint32_t t = lds[c * WAVES + w];
+      lds[c * WAVES + w] = sum;
+      sum += t;
+    }
+    lds[NC * WAVES + c] = sum ? atomicAdd(&counters[c], sum) : 0u;
+  }
+  __syncthreads();
+  if (cls < (uint32_t)NC) queues[cls][lds[NC * WAVES + cls] + lds[cls * WAVES + wave] + rank] = item;
+  __syncthreads();
+}
+
 /* -------------------------------------------------- primitive tests --- */
 /* Sphere::hit (objects/sphere.rs:70-125): strict t bounds, near root first */
 __device__ __forceinline__ bool sphere_t(V c, float r, V o, V d, float tmin, float tmax, float &t) {
@@ -379,12 +414,160 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
   }
 }
 
+/* ------------------------------------------- LDS-cached traversal --- */
+/* The same one-ray-per-thread traversal, with the top of the scene in LDS.
+ * Incoherent rays make every lane's node load hit a different cache line, and
+ * the vector-memory pipeline serialises such loads; LDS serves them at
+ * bank speed.  Each block copies the first `lds_nodes` wide nodes (the tree
+ * is numbered breadth-first, so this is its top) and the first `lds_units`
+ * float4 of primitive records into LDS, then traverses.  768-thread blocks
+ * (12 waves) amortise the copy; the per-lane stack keeps 8 entries in LDS
+ * and the rest in a global per-thread slice. */
+#define LT_BLOCK 768
+#define LT_STACK_LDS 8
+#define LT_STACK_GLOBAL (STACK_TOTAL - LT_STACK_LDS)
+
+struct LStack {
+  uint32_t *lst;   /* LDS, stride LT_BLOCK */
+  uint32_t *gst;   /* global slice */
+};
+__device__ __forceinline__ void ls_push(Lane &L, const LStack &S, uint32_t w) {
+  if (L.sp < LT_STACK_LDS) S.lst[(uint32_t)L.sp * LT_BLOCK] = w;
+  else if (L.sp < STACK_TOTAL) S.gst[L.sp - LT_STACK_LDS] = w;
+  else return; /* unreachable: the host bounds the BVH depth */
+  L.sp++;
+}
+__device__ __forceinline__ void ls_pop(Lane &L, const LStack &S) {
+  if (L.sp == 0) { L.word = CHILD_EMPTY; return; }
+  L.sp--;
+  L.word = L.sp < LT_STACK_LDS ? S.lst[(uint32_t)L.sp * LT_BLOCK] : S.gst[L.sp - LT_STACK_LDS];
+}
+
+/* one interior node whose 8 float4 start at nf (LDS or global) */
+__device__ __forceinline__ void lnode_at(Lane &L, const float4 *nf, const LStack &S) {
+  float4 lx = nf[0], hx = nf[1], ly = nf[2], hy = nf[3], lz = nf[4], hz = nf[5];
+  float4 cf = nf[6];
+  float tn[4];
+  uint32_t cw[4] = {as_u(cf.x), as_u(cf.y), as_u(cf.z), as_u(cf.w)};
+  float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+  float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+  float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+  const float tfar_lim = L.tb * 1.00000024f;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    float x0 = fmaf(lxa[k], L.inv.x, -L.oi.x), x1 = fmaf(hxa[k], L.inv.x, -L.oi.x);
+    float y0 = fmaf(lya[k], L.inv.y, -L.oi.y), y1 = fmaf(hya[k], L.inv.y, -L.oi.y);
+    float z0 = fmaf(lza[k], L.inv.z, -L.oi.z), z1 = fmaf(hza[k], L.inv.z, -L.oi.z);
+    float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), EPS_ACNE));
+    float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
+    bool hit = (tnear <= tfar) && (cw[k] != CHILD_EMPTY);
+    tn[k] = hit ? tnear : FLT_MAX;
+  }
+#define CSWAP(a, b)                                                       \
+  if (tn[b] < tn[a]) {                                                    \
+    float tt_ = tn[a]; tn[a] = tn[b]; tn[b] = tt_;                        \
+    uint32_t cc_ = cw[a]; cw[a] = cw[b]; cw[b] = cc_;                     \
+  }
+  CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+#undef CSWAP
+#pragma unroll
+  for (int k = 3; k >= 1; k--)
+    if (tn[k] < FLT_MAX) ls_push(L, S, cw[k]);
+  if (tn[0] < FLT_MAX) L.word = cw[0];
+  else ls_pop(L, S);
+}
+
+/* one primitive record at rec (LDS or global) */
+__device__ __forceinline__ void lprim_at(Lane &L, const DevScene &sc, const float4 *rec, uint32_t off, bool any_xform,
+                                         uint32_t &units) {
+  uint32_t hdr = as_u(rec[0].w);
+  float t;
+  bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t)
+                     : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t);
+  if (h) {
+    L.tb = t;
+    L.best = off;
+  }
+  units = prim_units(hdr & 3u);
+}
+
+template <bool COUNT, int MODE>
+__global__ __launch_bounds__(LT_BLOCK) void k_trace_lds(DevScene sc, PathBuf pb, TraceArgs ta) {
+  extern __shared__ float4 smem[];
+  const uint32_t nn = ta.lds_nodes, nu = ta.lds_units;
+  float4 *cnodes = smem;
+  float4 *cprims = smem + nn * 8u;
+  LStack S;
+  S.lst = reinterpret_cast<uint32_t *>(cprims + nu) + threadIdx.x;
+  S.gst = ta.spill + (size_t)(blockIdx.x * LT_BLOCK + threadIdx.x) * LT_STACK_GLOBAL;
+  {
+    const float4 *gn = reinterpret_cast<const float4 *>(sc.nodes);
+    for (uint32_t k = threadIdx.x; k < nn * 8u; k += LT_BLOCK) cnodes[k] = gn[k];
+    for (uint32_t k = threadIdx.x; k < nu; k += LT_BLOCK) cprims[k] = sc.prims[k];
+  }
+  __syncthreads();
+  __shared__ uint32_t agg[MC_COUNT * (LT_BLOCK / 64) + MC_COUNT];
+  const uint32_t n = MODE == 0 ? ta.counts[ta.in_slot] : ta.n_direct;
+  const bool any_xform = sc.has_xforms != 0;
+  uint32_t nvisit = 0, ntest = 0;
+  for (uint32_t base = blockIdx.x * LT_BLOCK; base < n; base += gridDim.x * LT_BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    const bool active = i < n;
+    Lane L;
+    L.live = false;
+    L.best = MISS_WORD;
+    L.path = 0;
+    if (active) lane_start(L, pb, MODE == 0 ? ta.qin[i] : i);
+    else L.word = CHILD_EMPTY;
+    while (L.word != CHILD_EMPTY) {
+      if (!(L.word & CHILD_LEAF)) {
+        if (L.word < nn) lnode_at(L, cnodes + L.word * 8u, S);
+        else lnode_at(L, reinterpret_cast<const float4 *>(sc.nodes + L.word), S);
+        if (COUNT) nvisit++;
+      } else {
+        uint32_t cnt = ((L.word >> 28) & 7u) + 1u;
+        uint32_t off = L.word & LEAF_OFFSET_MASK;
+        for (uint32_t k = 0; k < cnt; k++) {
+          uint32_t units;
+          if (off + PREC_QUAD <= nu) lprim_at(L, sc, cprims + off, off, any_xform, units);
+          else lprim_at(L, sc, sc.prims + off, off, any_xform, units);
+          off += units;
+        }
+        if (COUNT) ntest += cnt;
+        ls_pop(L, S);
+      }
+    }
+    if (MODE == 0) {
+      uint32_t cls = MC_COUNT;
+      if (active) {
+        pb.hit_t[L.path] = L.tb;
+        pb.hit_p[L.path] = L.best;
+        cls = L.best != MISS_WORD ? sc.mats[sc.info[as_u(sc.prims[L.best + 1u].w)].material].mclass : MC_MISS;
+      }
+      block_append<MC_COUNT, LT_BLOCK>(cls, L.path, ta.sq.q, ta.sq.counts, agg);
+    } else if (active) {
+      if (L.best == MISS_WORD) {
+        ta.ids[L.path] = 0xFFFFFFFFu;
+        ta.ts[L.path] = INFINITY;
+      } else {
+        ta.ids[L.path] = sc.info[as_u(sc.prims[L.best + 1u].w)].report_id;
+        ta.ts[L.path] = L.tb;
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(ta.stats + 0, (unsigned long long)nvisit);
+    atomicAdd(ta.stats + 1, (unsigned long long)ntest);
+  }
+}
+
 /* Closest-hit kernel, one ray per thread: the block's threads take rays
  * base + threadIdx.x in a grid-stride loop; each thread runs its ray to
  * completion (node or leaf per step).  Results as in k_trace below. */
-template <bool COUNT, int MODE>
+template <bool COUNT, int MODE, bool WW>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathBuf pb, TraceArgs ta) {
   __shared__ uint32_t lds_stack[STACK_LDS * TRACE_BLOCK];
+  __shared__ uint32_t agg[MC_COUNT * (TRACE_BLOCK / 64) + MC_COUNT];
   const uint32_t n = MODE == 0 ? ta.counts[ta.in_slot] : ta.n_direct;
   uint32_t *lst = lds_stack + threadIdx.x;
   uint32_t *gst = ta.spill + (size_t)(blockIdx.x * TRACE_BLOCK + threadIdx.x) * STACK_GLOBAL;
@@ -397,8 +580,25 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathB
     L.live = false;
     L.best = MISS_WORD;
     L.path = 0;
-    if (active) {
-      lane_start(L, pb, MODE == 0 ? ta.qin[i] : i);
+    if (active) lane_start(L, pb, MODE == 0 ? ta.qin[i] : i);
+    else L.word = CHILD_EMPTY;
+    if (WW) {
+      /* while-while: the wave descends interior nodes until every lane holds
+       * a leaf (or is done), then all lanes test their leaf together */
+      while (__any(L.word != CHILD_EMPTY)) {
+        while (__any(!(L.word & CHILD_LEAF))) {
+          if (!(L.word & CHILD_LEAF)) {
+            lane_node(L, sc, lst, gst);
+            if (COUNT) nvisit++;
+          }
+        }
+        if (L.word != CHILD_EMPTY) {
+          uint32_t c = lane_leaf(L, sc, any_xform);
+          if (COUNT) ntest += c;
+          lane_pop(L, lst, gst);
+        }
+      }
+    } else {
       while (L.word != CHILD_EMPTY) {
         if (!(L.word & CHILD_LEAF)) {
           lane_node(L, sc, lst, gst);
@@ -411,14 +611,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathB
       }
     }
     if (MODE == 0) {
-      uint32_t cls = MC_MISS;
+      uint32_t cls = MC_COUNT;
       if (active) {
         pb.hit_t[L.path] = L.tb;
         pb.hit_p[L.path] = L.best;
-        if (L.best != MISS_WORD) cls = sc.mats[sc.info[as_u(sc.prims[L.best + 1u].w)].material].mclass;
+        cls = L.best != MISS_WORD ? sc.mats[sc.info[as_u(sc.prims[L.best + 1u].w)].material].mclass : MC_MISS;
       }
-#pragma unroll
-      for (uint32_t c = 0; c < MC_COUNT; c++) wave_append(active && cls == c, L.path, ta.sq.q[c], ta.sq.counts + c);
+      block_append<MC_COUNT, TRACE_BLOCK>(cls, L.path, ta.sq.q, ta.sq.counts, agg);
     } else if (active) {
       if (L.best == MISS_WORD) {
         ta.ids[L.path] = 0xFFFFFFFFu;
@@ -840,6 +1039,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
 
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQueues sq, uint32_t *qout,
                                                 uint32_t *counts, uint32_t out_slot, uint32_t max_depth) {
+  __shared__ uint32_t agg[1 * 4 + 1];
   uint32_t c[MC_COUNT], chunks[MC_COUNT + 1];
   chunks[0] = 0;
 #pragma unroll
@@ -848,29 +1048,32 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
     chunks[k + 1] = chunks[k] + (c[k] + 63u) / 64u;
   }
   const uint32_t total = chunks[MC_COUNT];
-  const uint32_t waves_per_block = blockDim.x / 64u;
   const uint32_t lane = threadIdx.x & 63u;
-  for (uint32_t wv = blockIdx.x * waves_per_block + threadIdx.x / 64u; __builtin_amdgcn_readfirstlane(wv) < total;
-       wv += gridDim.x * waves_per_block) {
-    uint32_t cls = 0;
-#pragma unroll
-    for (uint32_t k = 1; k < MC_COUNT; k++)
-      if (wv >= chunks[k]) cls = k;
-    cls = __builtin_amdgcn_readfirstlane(cls);
-    uint32_t idx = (wv - chunks[cls]) * 64u + lane;
-    bool active = idx < c[cls];
-    uint32_t path = active ? sq.q[cls][idx] : 0u;
+  uint32_t *const out_q[1] = {qout};
+  /* block-uniform loop: each iteration the block's 4 waves take 4 consecutive 64-path chunks */
+  for (uint32_t first = blockIdx.x * 4u; first < total; first += gridDim.x * 4u) {
+    const uint32_t wv = first + (threadIdx.x >> 6);
+    uint32_t path = 0;
     bool cont = false;
-    if (active) {
-      switch (cls) {
-        case MC_MISS: cont = shade_path<MC_MISS>(sc, pb, path, max_depth); break;
-        case MC_LIGHT: cont = shade_path<MC_LIGHT>(sc, pb, path, max_depth); break;
-        case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN>(sc, pb, path, max_depth); break;
-        case MC_METAL: cont = shade_path<MC_METAL>(sc, pb, path, max_depth); break;
-        default: cont = shade_path<MC_GLASS>(sc, pb, path, max_depth); break;
+    if (wv < total) {
+      uint32_t cls = 0;
+#pragma unroll
+      for (uint32_t k = 1; k < MC_COUNT; k++)
+        if (wv >= chunks[k]) cls = k;
+      cls = __builtin_amdgcn_readfirstlane(cls);
+      uint32_t idx = (wv - chunks[cls]) * 64u + lane;
+      if (idx < c[cls]) {
+        path = sq.q[cls][idx];
+        switch (cls) {
+          case MC_MISS: cont = shade_path<MC_MISS>(sc, pb, path, max_depth); break;
+          case MC_LIGHT: cont = shade_path<MC_LIGHT>(sc, pb, path, max_depth); break;
+          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN>(sc, pb, path, max_depth); break;
+          case MC_METAL: cont = shade_path<MC_METAL>(sc, pb, path, max_depth); break;
+          default: cont = shade_path<MC_GLASS>(sc, pb, path, max_depth); break;
+        }
       }
     }
-    wave_append(cont, path, qout, counts + out_slot);
+    block_append<1, 256>(cont ? 0u : 1u, path, out_q, counts + out_slot, agg);
   }
 }
 
@@ -959,7 +1162,7 @@ uint32_t trace_grid_blocks() {
   int dev = 0, cus = 0, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_trace<false, MODE_EXTEND>),
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_trace_simple<false, MODE_EXTEND, false>),
                                                       TRACE_BLOCK, 0);
   if (cus < 1) cus = 1;
   if (per_cu < 1) per_cu = 1;
@@ -969,6 +1172,33 @@ uint32_t trace_grid_blocks() {
 
 size_t trace_spill_words(uint32_t blocks) { return (size_t)blocks * TRACE_BLOCK * STACK_GLOBAL; }
 
+size_t lds_trace_bytes(uint32_t nodes, uint32_t units) {
+  return (size_t)nodes * sizeof(Bvh4Node) + (size_t)units * 16u + (size_t)LT_STACK_LDS * LT_BLOCK * 4u;
+}
+uint32_t lds_trace_stack_bytes() { return LT_STACK_LDS * LT_BLOCK * 4u; }
+size_t lds_trace_spill_words(uint32_t blocks) { return (size_t)blocks * LT_BLOCK * LT_STACK_GLOBAL; }
+
+uint32_t lds_trace_grid_blocks(size_t lds_bytes) {
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_trace_lds<false, 0>),
+                                                      LT_BLOCK, lds_bytes);
+  if (cus < 1) cus = 1;
+  if (per_cu < 1) per_cu = 1;
+  return (uint32_t)(cus * per_cu);
+}
+
+void launch_trace_lds(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
+                      size_t lds_bytes, bool counting, int mode) {
+  if (mode == 1)
+    hipLaunchKernelGGL((k_trace_lds<false, 1>), dim3(blocks), dim3(LT_BLOCK), lds_bytes, st, sc, pb, ta);
+  else if (counting)
+    hipLaunchKernelGGL((k_trace_lds<true, 0>), dim3(blocks), dim3(LT_BLOCK), lds_bytes, st, sc, pb, ta);
+  else
+    hipLaunchKernelGGL((k_trace_lds<false, 0>), dim3(blocks), dim3(LT_BLOCK), lds_bytes, st, sc, pb, ta);
+}
+
 void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                    bool counting, int variant) {
   if (variant == TRACE_PERSISTENT) {
@@ -976,11 +1206,16 @@ void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const 
       hipLaunchKernelGGL((k_trace<true, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
     else
       hipLaunchKernelGGL((k_trace<false, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+  } else if (variant == TRACE_WW) {
+    if (counting)
+      hipLaunchKernelGGL((k_trace_simple<true, MODE_EXTEND, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+    else
+      hipLaunchKernelGGL((k_trace_simple<false, MODE_EXTEND, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
   } else {
     if (counting)
-      hipLaunchKernelGGL((k_trace_simple<true, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+      hipLaunchKernelGGL((k_trace_simple<true, MODE_EXTEND, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
     else
-      hipLaunchKernelGGL((k_trace_simple<false, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+      hipLaunchKernelGGL((k_trace_simple<false, MODE_EXTEND, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
   }
 }
 
@@ -989,7 +1224,7 @@ void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const
   if (variant == TRACE_PERSISTENT)
     hipLaunchKernelGGL((k_trace<false, MODE_PRIMARY>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
   else
-    hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+    hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
 }
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
                   uint32_t *counts, uint32_t out_slot, uint32_t max_depth, uint32_t max_paths, uint32_t grid_cap) {
