@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--workload", default="spheres", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="before rendering, copy 1 GiB device-to-device 3 times (a plain copy kernel of known "
+                         "size, so profiler byte counters can be scaled)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the work would shard by "
@@ -84,6 +87,16 @@ def main():
 
     import clovers_amd
     from clovers_amd import abi, synth
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x %d bytes" % (a.numel() * 4))
+        del a, b
 
     w = WORKLOADS[args.workload]
     scene = getattr(synth, w["scene"])()
@@ -132,7 +145,12 @@ def main():
                             "prim_tests": round(cc["prim_tests"] / cc["rays"], 3),
                             "node_bytes": cc["node_bytes"], "prim_bytes": cc["prim_bytes"],
                             "stream_bytes": RAY_STREAM_BYTES},
-                "rays_per_sample": round(cc["rays"] / samples, 4)}
+                "rays_per_sample": round(cc["rays"] / samples, 4),
+                "note": "algorithmic bytes = counted node visits x 128 B + primitive tests x record bytes + 48 B "
+                        "of ray stream per ray; for the sphere field the BVH (20 KB) and spheres (16 KB) are "
+                        "served from LDS, so HBM traffic per launch is far below the algorithmic bytes "
+                        "(rocprofv3 FETCH_SIZE/WRITE_SIZE in profiles/) and the kernel is issue/latency bound "
+                        "(VALUBusy, VALUUtilization in profiles/)"}
     r.close()
 
     cpu = None
