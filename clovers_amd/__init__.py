@@ -51,6 +51,8 @@ def load_library(path=LIB_PATH):
     lib.clv_scene_free.argtypes = [vp, C.c_uint32]
     lib.clv_render.restype = C.c_int
     lib.clv_render.argtypes = [vp, C.c_uint32, vp, vp, vp]
+    lib.clv_render_batch.restype = C.c_int
+    lib.clv_render_batch.argtypes = [vp, vp, C.c_uint32]
     lib.clv_primary_hits.restype = C.c_int
     lib.clv_primary_hits.argtypes = [vp, C.c_uint32, vp, vp, vp]
     lib.clv_counters_get.restype = C.c_int
@@ -143,6 +145,28 @@ class Renderer:
         self._check(self.lib.clv_render(self.h, scene_id, C.byref(opts), out.ctypes.data,
                                         sq.ctypes.data if sq is not None else None))
         return out, sq
+
+    def render_batch(self, jobs, want_sq=False):
+        """clv_render_batch.  jobs: list of (scene_id, opts) or (scene_id, opts,
+        camera) with camera an abi.CameraInit replacing the scene's camera.
+        Returns [(xyz[H, W, 3], sq or None)] in job order."""
+        arr = (abi.Job * len(jobs))()
+        outs = []
+        for k, jb in enumerate(jobs):
+            sid, opts = jb[0], jb[1]
+            cam = jb[2] if len(jb) > 2 else None
+            out = np.zeros((opts.height, opts.width, 3), np.float32)
+            sq = np.zeros((opts.height, opts.width, 3), np.float32) if want_sq else None
+            arr[k].scene_id = sid
+            arr[k].use_camera = 1 if cam is not None else 0
+            if cam is not None:
+                arr[k].camera = cam
+            arr[k].opts = opts
+            arr[k].xyz_out = out.ctypes.data
+            arr[k].xyz_sq_out = sq.ctypes.data if sq is not None else None
+            outs.append((out, sq))
+        self._check(self.lib.clv_render_batch(self.h, arr, len(jobs)))
+        return outs
 
     def primary_hits(self, scene_id, opts):
         W, H = opts.width, opts.height

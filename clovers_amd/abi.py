@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 NO_XFORM = 0xFFFFFFFF
 MISS = 0xFFFFFFFF
 
@@ -69,6 +69,15 @@ class RenderOpts(C.Structure):
                 ("guiding", C.c_uint32), ("fp64_accum", C.c_uint32), ("flags", C.c_uint32)]
 
 
+class Job(C.Structure):
+    """clv_job: one view of one scene in a batched render."""
+    _fields_ = [("scene_id", C.c_uint32), ("use_camera", C.c_uint32), ("camera", CameraInit),
+                ("_pad", C.c_uint32), ("opts", RenderOpts), ("xyz_out", C.c_void_p), ("xyz_sq_out", C.c_void_p)]
+
+
+assert C.sizeof(Job) == 128, C.sizeof(Job)
+
+
 class Counters(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("prim_tests", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),
@@ -85,6 +94,12 @@ class SceneInfo(C.Structure):
 
 FLAG_TIME_KERNELS = 1
 FLAG_COUNT = 2
+
+
+def camera_init(look_from, look_at, up=(0.0, 1.0, 0.0), vertical_fov=40.0, aperture=0.0, focus_distance=10.0,
+                time_0=0.0, time_1=1.0):
+    return CameraInit((C.c_float * 3)(*look_from), (C.c_float * 3)(*look_at), (C.c_float * 3)(*up),
+                      vertical_fov, aperture, focus_distance, time_0, time_1)
 
 
 def render_opts(width, height, samples, max_depth, seed=1, sample_offset=0, guiding=0, fp64_accum=0, flags=0):

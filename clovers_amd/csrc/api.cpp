@@ -34,14 +34,21 @@ using namespace clv;
 
 namespace {
 
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
+/* one growable device array of the scene arena */
+struct DevArray {
+  char *p = nullptr;
+  size_t used = 0, cap = 0;  /* bytes */
 };
 
+enum { AR_NODES, AR_PRIMS, AR_INFO, AR_XFORMS, AR_MATS, AR_SPD, AR_LIGHTS, AR_LNODES, AR_LWEIGHTS, AR_COUNT };
+
 struct SceneRes {
-  DevScene base;
-  std::vector<DevBuf> bufs;
+  ArenaBase base;            /* element offsets of the scene in the arena */
+  size_t end[AR_COUNT];      /* byte end of the scene in each array */
+  uint32_t light_root = LT_EMPTY;
+  uint32_t n_lights = 0;
+  int32_t bg_spd = -1;
+  bool has_xforms = false;
   clv_camera_init camera;
   uint32_t n_nodes = 0, n_prims = 0, prim_bytes = 0, depth = 0;
   uint32_t prim_units = 0;   /* float4 units of primitive records */
@@ -56,6 +63,7 @@ struct clv_ctx {
   std::string err;
   std::map<uint32_t, SceneRes *> scenes;
   uint32_t next_scene = 1;
+  DevArray arena[AR_COUNT];
   /* path state */
   uint32_t cap_paths = 0;
   PathBuf pb;
@@ -68,6 +76,11 @@ struct clv_ctx {
   float *out = nullptr, *out_sq = nullptr;
   uint32_t *ids = nullptr;
   float *ts = nullptr;
+  /* job and pass-item tables of the current render */
+  DevJob *jobs = nullptr;
+  uint32_t cap_jobs = 0;
+  DevItem *items = nullptr;
+  uint32_t cap_items = 0;
   /* misc */
   uint32_t *counts = nullptr;
   uint64_t *stats = nullptr;
@@ -100,6 +113,12 @@ clv_status fail(clv_ctx *c, clv_status s, const std::string &msg) {
     }                                                                                       \
   } while (0)
 
+#define CHK(expr)                       \
+  do {                                  \
+    clv_status s_ = (expr);             \
+    if (s_ != CLV_OK) return s_;        \
+  } while (0)
+
 template <typename T>
 hipError_t dalloc(T **p, size_t n) {
   return hipMalloc((void **)p, n * sizeof(T) + 16);
@@ -130,16 +149,19 @@ clv_status ensure_spill(clv_ctx *c, size_t words) {
   return CLV_OK;
 }
 
-/* traversal launch geometry of a scene for the configured variant */
+/* traversal launch geometry of a render for the configured variant; the LDS
+ * window caches the top of scene `s` */
 struct TraceGeom {
   uint32_t blocks = 0;
   size_t lds_bytes = 0;
-  uint32_t lds_nodes = 0, lds_units = 0;
+  uint32_t lds_node_base = 0, lds_nodes = 0, lds_unit_base = 0, lds_units = 0;
 };
 
 clv_status trace_geometry(clv_ctx *c, const SceneRes *s, uint32_t full_grid_paths, TraceGeom *g) {
   if (c->trace_variant == TRACE_LDS) {
     size_t budget = c->lds_block_bytes > lds_trace_stack_bytes() ? c->lds_block_bytes - lds_trace_stack_bytes() : 0;
+    g->lds_node_base = s->base.nodes;
+    g->lds_unit_base = s->base.units;
     g->lds_nodes = (uint32_t)std::min<size_t>(s->n_nodes, budget / sizeof(Bvh4Node));
     budget -= (size_t)g->lds_nodes * sizeof(Bvh4Node);
     g->lds_units = (uint32_t)std::min<size_t>(s->prim_units, budget / 16u);
@@ -150,6 +172,13 @@ clv_status trace_geometry(clv_ctx *c, const SceneRes *s, uint32_t full_grid_path
   g->blocks = c->trace_blocks;
   if (full_grid_paths && c->full_grid && c->trace_variant != TRACE_PERSISTENT) g->blocks = (full_grid_paths + 255u) / 256u;
   return ensure_spill(c, trace_spill_words(g->blocks));
+}
+
+void fill_trace_args(TraceArgs *ta, const TraceGeom &g) {
+  ta->lds_node_base = g.lds_node_base;
+  ta->lds_nodes = g.lds_nodes;
+  ta->lds_unit_base = g.lds_unit_base;
+  ta->lds_units = g.lds_units;
 }
 
 clv_status ensure_paths(clv_ctx *c, uint32_t n) {
@@ -189,19 +218,63 @@ clv_status ensure_pixels(clv_ctx *c, uint32_t n) {
   return CLV_OK;
 }
 
-void free_scene(SceneRes *s) {
-  for (auto &b : s->bufs) dfree(b.p);
-  delete s;
+clv_status ensure_tables(clv_ctx *c, uint32_t jobs, uint32_t items) {
+  if (jobs > c->cap_jobs) {
+    dfree(c->jobs);
+    c->jobs = nullptr;
+    c->cap_jobs = 0;
+    HIPCHK(c, dalloc(&c->jobs, jobs));
+    c->cap_jobs = jobs;
+  }
+  if (items > c->cap_items) {
+    dfree(c->items);
+    c->items = nullptr;
+    c->cap_items = 0;
+    HIPCHK(c, dalloc(&c->items, items));
+    c->cap_items = items;
+  }
+  return CLV_OK;
 }
 
-clv_status upload(clv_ctx *c, SceneRes *s, const void *src, size_t bytes, const void **dst) {
-  DevBuf b;
-  b.bytes = bytes ? bytes : 16;
-  HIPCHK(c, hipMalloc(&b.p, b.bytes));
-  s->bufs.push_back(b);
-  if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
-  *dst = b.p;
+/* append `bytes` to an arena array, growing it (a device-to-device copy of
+ * what it holds) when needed */
+clv_status arena_append(clv_ctx *c, DevArray &a, const void *src, size_t bytes) {
+  const size_t need = a.used + bytes;
+  if (need > a.cap) {
+    size_t cap = std::max<size_t>(std::max<size_t>(need, 2 * a.cap), 1u << 16);
+    char *p = nullptr;
+    HIPCHK(c, hipMalloc((void **)&p, cap));
+    if (a.used) {
+      hipError_t e = hipMemcpyAsync(p, a.p, a.used, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+        (void)hipFree(p);
+        return fail(c, CLV_EDEVICE, std::string("arena growth: ") + hipGetErrorString(e));
+      }
+    }
+    dfree(a.p);
+    a.p = p;
+    a.cap = cap;
+  }
+  if (bytes) HIPCHK(c, hipMemcpyAsync(a.p + a.used, src, bytes, hipMemcpyHostToDevice, c->stream));
+  a.used = need;
   return CLV_OK;
+}
+
+DevScene arena_view(const clv_ctx *c) {
+  DevScene d;
+  memset(&d, 0, sizeof(d));
+  d.nodes = (const Bvh4Node *)c->arena[AR_NODES].p;
+  d.prims = (const float4 *)c->arena[AR_PRIMS].p;
+  d.info = (const PrimInfo *)c->arena[AR_INFO].p;
+  d.xforms = (const Xform *)c->arena[AR_XFORMS].p;
+  d.mats = (const DevMaterial *)c->arena[AR_MATS].p;
+  d.spd = (const float *)c->arena[AR_SPD].p;
+  d.lights = (const DevLight *)c->arena[AR_LIGHTS].p;
+  d.lnodes = (const DevLightNode *)c->arena[AR_LNODES].p;
+  d.lweights = (const float *)c->arena[AR_LWEIGHTS].p;
+  d.jobs = c->jobs;
+  return d;
 }
 
 clv_status check_opts(clv_ctx *c, const clv_render_opts *o) {
@@ -213,10 +286,228 @@ clv_status check_opts(clv_ctx *c, const clv_render_opts *o) {
   return CLV_OK;
 }
 
-DevScene scene_for(const SceneRes *s, const clv_render_opts *o) {
-  DevScene d = s->base;
-  d.cam = make_camera(s->camera, (float)o->width / (float)o->height);
-  return d;
+DevJob make_job(const SceneRes *s, const clv_camera_init &cam, const clv_render_opts *o) {
+  DevJob j;
+  memset(&j, 0, sizeof(j));
+  j.cam = make_camera(cam, (float)o->width / (float)o->height);
+  j.root = s->base.nodes;
+  j.light_root = s->light_root;
+  j.light_base = s->base.lights;
+  j.n_lights = s->n_lights;
+  j.bg_spd = s->bg_spd;
+  j.width = o->width;
+  j.height = o->height;
+  j.max_depth = o->max_depth;
+  j.seed = o->seed;
+  j.spp = o->samples;
+  j.fp64 = o->fp64_accum ? 1u : 0u;
+  j.guiding = o->guiding;
+  return j;
+}
+
+/* Split the jobs' samples into wavefront passes of about `target` paths: a
+ * pass holds, in job order, as many whole-sample slices (every pixel of a job,
+ * `samples` samples each) as fit; a job larger than the target still gets one
+ * sample per pass. */
+struct PassPlan {
+  std::vector<DevItem> items;       /* all passes, concatenated */
+  std::vector<uint32_t> first;      /* per pass: first item */
+  std::vector<uint32_t> paths;      /* per pass: paths */
+  std::vector<uint32_t> res;        /* per pass: resolve threads */
+  std::vector<uint32_t> depth;      /* per pass: largest max_depth of its jobs */
+  uint32_t max_paths = 0;
+};
+
+void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &sample_offset, uint32_t target,
+                 PassPlan *pl) {
+  const uint32_t n = (uint32_t)jobs.size();
+  std::vector<uint32_t> done(n, 0);
+  uint32_t j = 0;
+  while (j < n) {
+    uint32_t budget = target, paths = 0, res = 0, depth = 0;
+    pl->first.push_back((uint32_t)pl->items.size());
+    while (j < n) {
+      const uint32_t pix = jobs[j].width * jobs[j].height;
+      const uint32_t left = jobs[j].spp - done[j];
+      uint32_t kk = budget / pix;
+      if (kk == 0) {
+        if (paths) break;  /* start a new pass */
+        kk = 1;
+      }
+      if (kk > left) kk = left;
+      DevItem it;
+      memset(&it, 0, sizeof(it));
+      it.job = j;
+      it.path_base = paths;
+      it.pixels = pix;
+      it.samples = kk;
+      it.sample_base = sample_offset[j] + done[j];
+      it.res_base = res;
+      pl->items.push_back(it);
+      paths += kk * pix;
+      res += pix;
+      depth = std::max(depth, jobs[j].max_depth);
+      budget = budget > kk * pix ? budget - kk * pix : 0;
+      done[j] += kk;
+      if (done[j] == jobs[j].spp) j++;
+      if (budget == 0) break;
+    }
+    pl->paths.push_back(paths);
+    pl->res.push_back(res);
+    pl->depth.push_back(depth);
+    pl->max_paths = std::max(pl->max_paths, paths);
+  }
+  pl->first.push_back((uint32_t)pl->items.size());
+}
+
+/* the batch render behind clv_render and clv_render_batch */
+clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
+  if (n == 0 || !jobs) return fail(c, CLV_EINVAL, "no jobs");
+  if (n > MAX_JOBS) return fail(c, CLV_EINVAL, "more than 8192 jobs in one batch");
+  std::vector<DevJob> dj(n);
+  std::vector<uint32_t> soff(n);
+  std::vector<const SceneRes *> sres(n);
+  uint64_t pixels = 0, samples = 0;
+  uint32_t flags = 0;
+  bool any_xform = false;
+  for (uint32_t k = 0; k < n; k++) {
+    const clv_job &jb = jobs[k];
+    CHK(check_opts(c, &jb.opts));
+    if (!jb.xyz_out) return fail(c, CLV_EINVAL, "xyz_out is NULL");
+    if (jb.opts.samples == 0) return fail(c, CLV_EINVAL, "samples must be positive");
+    if (jb.opts.guiding == CLV_GUIDE_ON) return fail(c, CLV_EUNSUPPORTED, "guiding is not available in this build");
+    auto it = c->scenes.find(jb.scene_id);
+    if (it == c->scenes.end()) return fail(c, CLV_EINVAL, "unknown scene id");
+    sres[k] = it->second;
+    dj[k] = make_job(it->second, jb.use_camera ? jb.camera : it->second->camera, &jb.opts);
+    dj[k].pixel_base = (uint32_t)pixels;
+    soff[k] = jb.opts.sample_offset;
+    pixels += (uint64_t)jb.opts.width * jb.opts.height;
+    samples += (uint64_t)jb.opts.width * jb.opts.height * jb.opts.samples;
+    flags |= jb.opts.flags;
+    any_xform = any_xform || it->second->has_xforms;
+  }
+  if (pixels > (1ull << 28)) return fail(c, CLV_EINVAL, "batch larger than 2^28 pixels");
+  HIPCHK(c, hipSetDevice(c->device));
+  PassPlan pl;
+  plan_passes(dj, soff, c->target_paths, &pl);
+  const uint32_t passes = (uint32_t)pl.paths.size();
+  CHK(ensure_paths(c, pl.max_paths));
+  CHK(ensure_pixels(c, (uint32_t)pixels));
+  CHK(ensure_tables(c, n, (uint32_t)pl.items.size()));
+  /* the LDS window caches the scene with the most samples */
+  uint32_t big = 0;
+  for (uint32_t k = 1; k < n; k++)
+    if ((uint64_t)dj[k].width * dj[k].height * dj[k].spp > (uint64_t)dj[big].width * dj[big].height * dj[big].spp) big = k;
+  TraceGeom geom;
+  CHK(trace_geometry(c, sres[big], pl.max_paths, &geom));
+  HIPCHK(c, hipMemcpyAsync(c->jobs, dj.data(), n * sizeof(DevJob), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->items, pl.items.data(), pl.items.size() * sizeof(DevItem), hipMemcpyHostToDevice,
+                           c->stream));
+  DevScene ds = arena_view(c);
+  ds.has_xforms = any_xform ? 1u : 0u;
+  const bool timing = (flags & CLV_FLAG_TIME_KERNELS) != 0;
+  const bool counting = (flags & CLV_FLAG_COUNT) != 0;
+  uint32_t extends = 0;
+  for (uint32_t p = 0; p < passes; p++) extends += pl.depth[p] + 1u;
+  size_t need_events = 2 + (timing ? 2 * (size_t)extends : 0);
+  while (c->events.size() < need_events) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c->events.push_back(e);
+  }
+  ResolveParams rp;
+  memset(&rp, 0, sizeof(rp));
+  rp.jobs = c->jobs;
+  rp.n_jobs = n;
+  rp.pixels = (uint32_t)pixels;
+  rp.acc32 = c->acc32;
+  rp.acc64 = c->acc64;
+  rp.acc_sq = c->accsq;
+  ShadeQueues sq;
+  for (uint32_t q = 0; q < MC_COUNT; q++) sq.q[q] = c->shadeq[q];
+  sq.counts = c->counts + CNT_SHADE;
+  HIPCHK(c, hipMemsetAsync(c->stats, 0, 4 * sizeof(uint64_t), c->stream));
+  launch_zero_accum(c->stream, rp);
+  HIPCHK(c, hipEventRecord(c->events[0], c->stream));
+  size_t ev = 2;
+  for (uint32_t p = 0; p < passes; p++) {
+    GenParams gp;
+    memset(&gp, 0, sizeof(gp));
+    gp.items = c->items + pl.first[p];
+    gp.n_items = pl.first[p + 1] - pl.first[p];
+    gp.paths = pl.paths[p];
+    launch_set_count(c->stream, c->counts, CNT_QA, gp.paths);
+    launch_generate(c->stream, ds, c->pb, gp, c->queue[0]);
+    for (uint32_t depth = 0; depth <= pl.depth[p]; depth++) {
+      uint32_t in = depth & 1u, out = in ^ 1u;
+      uint32_t in_slot = in ? CNT_QB : CNT_QA, out_slot = out ? CNT_QB : CNT_QA;
+      launch_reset_counts(c->stream, c->counts, in_slot, out_slot, c->stats);
+      if (timing) HIPCHK(c, hipEventRecord(c->events[ev], c->stream));
+      TraceArgs ta;
+      memset(&ta, 0, sizeof(ta));
+      ta.qin = c->queue[in];
+      ta.counts = c->counts;
+      ta.in_slot = in_slot;
+      ta.sq = sq;
+      ta.spill = c->spill;
+      ta.stats = (unsigned long long *)c->stats;
+      fill_trace_args(&ta, geom);
+      if (c->trace_variant == TRACE_LDS)
+        launch_trace_lds(c->stream, ds, c->pb, ta, geom.blocks, geom.lds_bytes, counting, 0);
+      else
+        launch_extend(c->stream, ds, c->pb, ta, geom.blocks, counting, c->trace_variant);
+      if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
+      ev += timing ? 2 : 0;
+      launch_shade(c->stream, ds, c->pb, sq, c->queue[out], c->counts, out_slot, gp.paths, c->grid_cap);
+    }
+    rp.items = gp.items;
+    rp.n_items = gp.n_items;
+    rp.threads = pl.res[p];
+    launch_resolve(c->stream, c->pb, rp, c->cie);
+  }
+  bool any_sq = false;
+  for (uint32_t k = 0; k < n; k++) any_sq = any_sq || jobs[k].xyz_sq_out;
+  launch_finalize(c->stream, rp, c->out, any_sq ? c->out_sq : nullptr);
+  HIPCHK(c, hipEventRecord(c->events[1], c->stream));
+  HIPCHK(c, hipGetLastError());
+  for (uint32_t k = 0; k < n; k++) {
+    const size_t off = 3 * (size_t)dj[k].pixel_base, cnt = 3 * (size_t)dj[k].width * dj[k].height;
+    HIPCHK(c, hipMemcpyAsync(jobs[k].xyz_out, c->out + off, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->stream));
+    if (jobs[k].xyz_sq_out)
+      HIPCHK(c, hipMemcpyAsync(jobs[k].xyz_sq_out, c->out_sq + off, sizeof(float) * cnt, hipMemcpyDeviceToHost,
+                               c->stream));
+  }
+  uint64_t stats[4];
+  HIPCHK(c, hipMemcpyAsync(stats, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.0f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->events[0], c->events[1]));
+  clv_counters &cn = c->counters;
+  cn.samples = samples;
+  cn.rays = stats[2];
+  cn.node_visits = counting ? stats[0] : 0;
+  cn.prim_tests = counting ? stats[1] : 0;
+  cn.render_ms = ms;
+  cn.extend_ms = 0.0;
+  if (timing) {
+    for (size_t e = 2; e < ev; e += 2) {
+      float m = 0.0f;
+      HIPCHK(c, hipEventElapsedTime(&m, c->events[e], c->events[e + 1]));
+      cn.extend_ms += m;
+    }
+  }
+  const SceneRes *s = sres[big];
+  cn.extend_launches = extends;
+  cn.passes = passes;
+  cn.bvh_nodes = s->n_nodes;
+  cn.bvh_width = 4;
+  cn.node_bytes = sizeof(Bvh4Node);
+  cn.prim_bytes = s->prim_bytes;
+  cn.n_prims = s->n_prims;
+  cn.bvh_depth = s->depth;
+  cn.build_ms = s->build_ms;
+  return CLV_OK;
 }
 
 }  // namespace
@@ -286,9 +577,12 @@ void clv_destroy(clv_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto &kv : c->scenes) free_scene(kv.second);
+  for (auto &kv : c->scenes) delete kv.second;
+  for (auto &a : c->arena) dfree(a.p);
   free_paths(c);
   free_pixels(c);
+  dfree(c->jobs);
+  dfree(c->items);
   dfree(c->counts);
   dfree(c->stats);
   dfree(c->spill);
@@ -307,38 +601,54 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
   std::string err;
   int st = build_host_scene(desc, &hs, &err);
   if (st != CLV_OK) return fail(c, (clv_status)st, err);
+  ArenaBase b;
+  b.nodes = (uint32_t)(c->arena[AR_NODES].used / sizeof(Bvh4Node));
+  b.units = (uint32_t)(c->arena[AR_PRIMS].used / 16u);
+  b.info = (uint32_t)(c->arena[AR_INFO].used / sizeof(PrimInfo));
+  b.xforms = (uint32_t)(c->arena[AR_XFORMS].used / sizeof(Xform));
+  b.mats = (uint32_t)(c->arena[AR_MATS].used / sizeof(DevMaterial));
+  b.spd = (uint32_t)(c->arena[AR_SPD].used / sizeof(float));
+  b.lights = (uint32_t)(c->arena[AR_LIGHTS].used / sizeof(DevLight));
+  b.lnodes = (uint32_t)(c->arena[AR_LNODES].used / sizeof(DevLightNode));
+  st = rebase_host_scene(&hs, b, &err);
+  if (st != CLV_OK) return fail(c, (clv_status)st, err);
   SceneRes *s = new (std::nothrow) SceneRes();
   if (!s) return fail(c, CLV_ENOMEM, "host allocation failed");
-  memset(&s->base, 0, sizeof(s->base));
+  s->base = b;
+  size_t before[AR_COUNT];
+  for (int k = 0; k < AR_COUNT; k++) before[k] = c->arena[k].used;
+  struct Part {
+    int arr;
+    const void *p;
+    size_t bytes;
+  } parts[AR_COUNT] = {
+      {AR_NODES, hs.nodes.data(), hs.nodes.size() * sizeof(Bvh4Node)},
+      {AR_PRIMS, hs.prims.data(), hs.prims.size() * sizeof(float)},
+      {AR_INFO, hs.prim_info.data(), hs.prim_info.size() * sizeof(PrimInfo)},
+      {AR_XFORMS, hs.xforms.data(), hs.xforms.size() * sizeof(Xform)},
+      {AR_MATS, hs.materials.data(), hs.materials.size() * sizeof(DevMaterial)},
+      {AR_SPD, hs.spd.data(), hs.spd.size() * sizeof(float)},
+      {AR_LIGHTS, hs.lights.data(), hs.lights.size() * sizeof(DevLight)},
+      {AR_LNODES, hs.light_nodes.data(), hs.light_nodes.size() * sizeof(DevLightNode)},
+      {AR_LWEIGHTS, hs.light_weights.data(), hs.light_weights.size() * sizeof(float)},
+  };
   clv_status r = CLV_OK;
-  const void *p;
-#define UP(vec, field, T)                                                                      \
-  if (r == CLV_OK) {                                                                           \
-    r = upload(c, s, hs.vec.data(), hs.vec.size() * sizeof(hs.vec[0]), &p);                   \
-    s->base.field = (const T *)p;                                                              \
-  }
-  UP(nodes, nodes, Bvh4Node);
-  UP(prims, prims, float4);
-  UP(prim_info, info, PrimInfo);
-  UP(xforms, xforms, Xform);
-  UP(materials, mats, DevMaterial);
-  UP(spd, spd, float);
-  UP(lights, lights, DevLight);
-  UP(light_nodes, lnodes, DevLightNode);
-  UP(light_weights, lweights, float);
-#undef UP
+  for (const Part &pt : parts)
+    if (r == CLV_OK) r = arena_append(c, c->arena[pt.arr], pt.p, pt.bytes);
   if (r == CLV_OK) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) r = fail(c, CLV_EDEVICE, hipGetErrorString(e));
   }
-  if (r != CLV_OK) {
-    free_scene(s);
+  if (r != CLV_OK) { /* roll the arena back to where it was */
+    for (int k = 0; k < AR_COUNT; k++) c->arena[k].used = before[k];
+    delete s;
     return r;
   }
-  s->base.n_lights = (uint32_t)hs.lights.size();
-  s->base.light_root = hs.light_root;
-  s->base.bg_spd = hs.background_spd;
-  s->base.has_xforms = hs.xforms.empty() ? 0u : 1u;
+  for (int k = 0; k < AR_COUNT; k++) s->end[k] = c->arena[k].used;
+  s->n_lights = (uint32_t)hs.lights.size();
+  s->light_root = hs.light_root;
+  s->bg_spd = hs.background_spd;
+  s->has_xforms = !hs.xforms.empty();
   s->camera = hs.camera;
   s->n_nodes = (uint32_t)hs.nodes.size();
   s->prim_units = (uint32_t)(hs.prims.size() / 4);
@@ -378,170 +688,80 @@ clv_status clv_scene_validate(const clv_scene_desc *desc, clv_scene_info *info) 
   return CLV_OK;
 }
 
+/* A freed scene's arena range is reused only when it is the last one in the
+ * arena (then the arena shrinks back to the end of the previous live scene);
+ * freeing every scene empties the arena. */
 clv_status clv_scene_free(clv_ctx *c, uint32_t scene_id) {
   if (!c) return CLV_EINVAL;
   auto it = c->scenes.find(scene_id);
   if (it == c->scenes.end()) return fail(c, CLV_EINVAL, "unknown scene id");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  free_scene(it->second);
+  delete it->second;
   c->scenes.erase(it);
+  for (int k = 0; k < AR_COUNT; k++) {
+    size_t end = 0;
+    for (auto &kv : c->scenes) end = std::max(end, kv.second->end[k]);
+    c->arena[k].used = std::min(c->arena[k].used, end);
+  }
   return CLV_OK;
 }
 
 clv_status clv_render(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, float *xyz_out, float *xyz_sq_out) {
   if (!c) return CLV_EINVAL;
-  clv_status st = check_opts(c, o);
-  if (st != CLV_OK) return st;
-  if (!xyz_out) return fail(c, CLV_EINVAL, "xyz_out is NULL");
-  if (o->samples == 0) return fail(c, CLV_EINVAL, "samples must be positive");
-  if (o->guiding == CLV_GUIDE_ON) return fail(c, CLV_EUNSUPPORTED, "guiding is not available in this build");
-  auto it = c->scenes.find(scene_id);
-  if (it == c->scenes.end()) return fail(c, CLV_EINVAL, "unknown scene id");
-  const SceneRes *s = it->second;
-  HIPCHK(c, hipSetDevice(c->device));
-  const uint32_t W = o->width, H = o->height, WH = W * H;
-  uint32_t k = c->target_paths / WH;
-  if (k < 1) k = 1;
-  if (k > o->samples) k = o->samples;
-  if ((st = ensure_paths(c, WH * k)) != CLV_OK) return st;
-  if ((st = ensure_pixels(c, WH)) != CLV_OK) return st;
-  TraceGeom geom;
-  if ((st = trace_geometry(c, s, WH * k, &geom)) != CLV_OK) return st;
-  const DevScene ds = scene_for(s, o);
-  const bool timing = (o->flags & CLV_FLAG_TIME_KERNELS) != 0;
-  const bool counting = (o->flags & CLV_FLAG_COUNT) != 0;
-  const uint32_t passes = (o->samples + k - 1) / k;
-  const uint32_t extends = passes * (o->max_depth + 1);
-  size_t need_events = 2 + (timing ? 2 * (size_t)extends : 0);
-  while (c->events.size() < need_events) {
-    hipEvent_t e;
-    HIPCHK(c, hipEventCreate(&e));
-    c->events.push_back(e);
-  }
-  ResolveParams rp;
-  memset(&rp, 0, sizeof(rp));
-  rp.pixels = WH;
-  rp.fp64 = o->fp64_accum ? 1u : 0u;
-  rp.acc32 = c->acc32;
-  rp.acc64 = c->acc64;
-  rp.acc_sq = c->accsq;
-  ShadeQueues sq;
-  for (uint32_t q = 0; q < MC_COUNT; q++) sq.q[q] = c->shadeq[q];
-  sq.counts = c->counts + CNT_SHADE;
-  HIPCHK(c, hipMemsetAsync(c->stats, 0, 4 * sizeof(uint64_t), c->stream));
-  launch_zero_accum(c->stream, rp);
-  HIPCHK(c, hipEventRecord(c->events[0], c->stream));
-  size_t ev = 2;
-  uint32_t done = 0;
-  for (uint32_t p = 0; p < passes; p++) {
-    uint32_t kk = o->samples - done < k ? o->samples - done : k;
-    GenParams gp;
-    memset(&gp, 0, sizeof(gp));
-    gp.width = W;
-    gp.height = H;
-    gp.pixels = WH;
-    gp.paths = WH * kk;
-    gp.sample_base = o->sample_offset + done;
-    gp.seed = o->seed;
-    launch_set_count(c->stream, c->counts, CNT_QA, gp.paths);
-    launch_generate(c->stream, ds, c->pb, gp, c->queue[0]);
-    for (uint32_t depth = 0; depth <= o->max_depth; depth++) {
-      uint32_t in = depth & 1u, out = in ^ 1u;
-      uint32_t in_slot = in ? CNT_QB : CNT_QA, out_slot = out ? CNT_QB : CNT_QA;
-      launch_reset_counts(c->stream, c->counts, in_slot, out_slot, c->stats);
-      if (timing) HIPCHK(c, hipEventRecord(c->events[ev], c->stream));
-      TraceArgs ta;
-      memset(&ta, 0, sizeof(ta));
-      ta.qin = c->queue[in];
-      ta.counts = c->counts;
-      ta.in_slot = in_slot;
-      ta.sq = sq;
-      ta.spill = c->spill;
-      ta.stats = (unsigned long long *)c->stats;
-      ta.lds_nodes = geom.lds_nodes;
-      ta.lds_units = geom.lds_units;
-      if (c->trace_variant == TRACE_LDS)
-        launch_trace_lds(c->stream, ds, c->pb, ta, geom.blocks, geom.lds_bytes, counting, 0);
-      else
-        launch_extend(c->stream, ds, c->pb, ta, geom.blocks, counting, c->trace_variant);
-      if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
-      ev += timing ? 2 : 0;
-      launch_shade(c->stream, ds, c->pb, sq, c->queue[out], c->counts, out_slot, o->max_depth, gp.paths, c->grid_cap);
-    }
-    rp.per_pixel = kk;
-    launch_resolve(c->stream, c->pb, rp, c->cie);
-    done += kk;
-  }
-  launch_finalize(c->stream, rp, c->out, xyz_sq_out ? c->out_sq : nullptr, o->samples);
-  HIPCHK(c, hipEventRecord(c->events[1], c->stream));
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(xyz_out, c->out, 3 * sizeof(float) * WH, hipMemcpyDeviceToHost, c->stream));
-  if (xyz_sq_out) HIPCHK(c, hipMemcpyAsync(xyz_sq_out, c->out_sq, 3 * sizeof(float) * WH, hipMemcpyDeviceToHost, c->stream));
-  uint64_t stats[4];
-  HIPCHK(c, hipMemcpyAsync(stats, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  float ms = 0.0f;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->events[0], c->events[1]));
-  clv_counters &cn = c->counters;
-  cn.samples = (uint64_t)WH * o->samples;
-  cn.rays = stats[2];
-  cn.node_visits = counting ? stats[0] : 0;
-  cn.prim_tests = counting ? stats[1] : 0;
-  cn.render_ms = ms;
-  cn.extend_ms = 0.0;
-  if (timing) {
-    for (size_t e = 2; e < ev; e += 2) {
-      float m = 0.0f;
-      HIPCHK(c, hipEventElapsedTime(&m, c->events[e], c->events[e + 1]));
-      cn.extend_ms += m;
-    }
-  }
-  cn.extend_launches = extends;
-  cn.passes = passes;
-  cn.bvh_nodes = s->n_nodes;
-  cn.bvh_width = 4;
-  cn.node_bytes = sizeof(Bvh4Node);
-  cn.prim_bytes = s->prim_bytes;
-  cn.n_prims = s->n_prims;
-  cn.bvh_depth = s->depth;
-  cn.build_ms = s->build_ms;
-  return CLV_OK;
+  CHK(check_opts(c, o));
+  clv_job j;
+  memset(&j, 0, sizeof(j));
+  j.scene_id = scene_id;
+  j.opts = *o;
+  j.xyz_out = xyz_out;
+  j.xyz_sq_out = xyz_sq_out;
+  return render_jobs(c, &j, 1);
+}
+
+clv_status clv_render_batch(clv_ctx *c, const clv_job *jobs, uint32_t n_jobs) {
+  if (!c) return CLV_EINVAL;
+  return render_jobs(c, jobs, n_jobs);
 }
 
 clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, uint32_t *ids, float *ts) {
   if (!c) return CLV_EINVAL;
-  clv_status st = check_opts(c, o);
-  if (st != CLV_OK) return st;
+  CHK(check_opts(c, o));
   if (!ids) return fail(c, CLV_EINVAL, "prim_id_out is NULL");
   auto it = c->scenes.find(scene_id);
   if (it == c->scenes.end()) return fail(c, CLV_EINVAL, "unknown scene id");
   HIPCHK(c, hipSetDevice(c->device));
   const uint32_t W = o->width, H = o->height, WH = W * H;
-  if ((st = ensure_paths(c, WH)) != CLV_OK) return st;
-  if ((st = ensure_pixels(c, WH)) != CLV_OK) return st;
-  const DevScene ds = scene_for(it->second, o);
+  CHK(ensure_paths(c, WH));
+  CHK(ensure_pixels(c, WH));
+  CHK(ensure_tables(c, 1, 1));
+  DevJob jb = make_job(it->second, it->second->camera, o);
+  DevItem item;
+  memset(&item, 0, sizeof(item));
+  item.pixels = WH;
+  item.samples = 1;
+  item.sample_base = o->sample_offset;
+  HIPCHK(c, hipMemcpyAsync(c->jobs, &jb, sizeof(jb), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->items, &item, sizeof(item), hipMemcpyHostToDevice, c->stream));
+  DevScene ds = arena_view(c);
+  ds.has_xforms = it->second->has_xforms ? 1u : 0u;
   GenParams gp;
   memset(&gp, 0, sizeof(gp));
-  gp.width = W;
-  gp.height = H;
-  gp.pixels = WH;
+  gp.items = c->items;
+  gp.n_items = 1;
   gp.paths = WH;
-  gp.sample_base = o->sample_offset;
-  gp.seed = o->seed;
   launch_generate(c->stream, ds, c->pb, gp, c->queue[0]);
   launch_set_count(c->stream, c->counts, CNT_FETCH, 0u);
   TraceArgs ta;
   memset(&ta, 0, sizeof(ta));
   ta.counts = c->counts;
   ta.n_direct = WH;
-  ta.spill = c->spill;
   ta.ids = c->ids;
   ta.ts = c->ts;
   TraceGeom geom;
-  if ((st = trace_geometry(c, it->second, 0, &geom)) != CLV_OK) return st;
-  ta.lds_nodes = geom.lds_nodes;
-  ta.lds_units = geom.lds_units;
+  CHK(trace_geometry(c, it->second, 0, &geom));
+  ta.spill = c->spill;
+  fill_trace_args(&ta, geom);
   if (c->trace_variant == TRACE_LDS)
     launch_trace_lds(c->stream, ds, c->pb, ta, geom.blocks, geom.lds_bytes, false, 1);
   else

@@ -884,4 +884,46 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
   return CLV_OK;
 }
 
+static uint32_t rebase_light_word(uint32_t w, const ArenaBase &b) {
+  if (w == LT_EMPTY) return w;
+  if (w & LT_LEAF) return LT_LEAF | ((w & ~LT_LEAF) + b.lights);
+  return w + b.lnodes;
+}
+
+int rebase_host_scene(HostScene *hs, const ArenaBase &b, std::string *err) {
+  const uint64_t units = hs->prims.size() / 4;
+  if ((uint64_t)b.units + units > (uint64_t)LEAF_OFFSET_MASK) {
+    *err = "device arena too large for 28-bit record offsets (free scenes first)";
+    return CLV_EUNSUPPORTED;
+  }
+  for (Bvh4Node &n : hs->nodes)
+    for (uint32_t &w : n.child) {
+      if (w == CHILD_EMPTY) continue;
+      if (w & CHILD_LEAF) w = (w & ~LEAF_OFFSET_MASK) | ((w & LEAF_OFFSET_MASK) + b.units);
+      else w += b.nodes;
+    }
+  float *rec = hs->prims.data();
+  for (uint64_t u = 0; u < units;) {
+    uint32_t hdr, info;
+    memcpy(&hdr, rec + 4 * u + 3, 4);
+    memcpy(&info, rec + 4 * (u + 1) + 3, 4);
+    const uint32_t type = hdr & 3u, xf = hdr >> 2;
+    if (xf) hdr = type | ((xf + b.xforms) << 2);
+    info += b.info;
+    memcpy(rec + 4 * u + 3, &hdr, 4);
+    memcpy(rec + 4 * (u + 1) + 3, &info, 4);
+    u += type == PT_SPHERE ? PREC_SPHERE : (type == PT_TRI ? PREC_TRI : PREC_QUAD);
+  }
+  for (PrimInfo &pi : hs->prim_info) pi.material += b.mats;
+  for (DevMaterial &m : hs->materials) {
+    if (m.albedo_spd >= 0) m.albedo_spd += (int32_t)b.spd;
+    if (m.emit_spd >= 0) m.emit_spd += (int32_t)b.spd;
+  }
+  if (hs->background_spd >= 0) hs->background_spd += (int32_t)b.spd;
+  for (DevLightNode &n : hs->light_nodes)
+    for (uint32_t &w : n.child) w = rebase_light_word(w, b);
+  hs->light_root = rebase_light_word(hs->light_root, b);
+  return CLV_OK;
+}
+
 }  // namespace clv

@@ -34,6 +34,17 @@ struct HostScene {
  * clv_status; *err holds the detail on failure. */
 int build_host_scene(const clv_scene_desc *desc, HostScene *out, std::string *err);
 
+/* Offsets of a scene inside the context's device arena, in elements of each
+ * array (prims and spd in float4 units / floats). */
+struct ArenaBase {
+  uint32_t nodes = 0, units = 0, info = 0, xforms = 0, mats = 0, spd = 0, lights = 0, lnodes = 0;
+};
+
+/* Rewrite every index of a built scene for placement at `b`: interior child
+ * words, leaf record offsets, record xform and info indices, materials, SPD
+ * offsets, light-tree words.  Fails if a leaf offset leaves 28 bits. */
+int rebase_host_scene(HostScene *hs, const ArenaBase &b, std::string *err);
+
 /* Camera::new (camera.rs:59-99) for the given aspect ratio. */
 DevCamera make_camera(const clv_camera_init &ci, float aspect);
 

@@ -17,6 +17,46 @@ namespace clv {
 #define CNT_FETCH 7u    /* work counter of the persistent traversal */
 #define CNT_TOTAL 8u
 
+/* path meta word: hero wavelength index | depth | job of the batch */
+#define META_HERO_MASK 0x1FFu
+#define META_DEPTH_SHIFT 9u
+#define META_DEPTH_MASK 0x3FFu   /* max_depth <= 1000 */
+#define META_JOB_SHIFT 19u
+#define MAX_JOBS 8192u           /* 13 bits of job index */
+
+/* per-job constants of a batch: one view of one scene */
+struct DevJob {
+  DevCamera cam;
+  uint32_t root;          /* BVH root node (arena index) */
+  uint32_t light_root;    /* light-tree root word (arena indices) */
+  uint32_t light_base;    /* the scene's MIS leaves: lights[light_base, +n_lights) */
+  uint32_t n_lights;
+  int32_t bg_spd;         /* background SPD offset in spd[] */
+  uint32_t width, height;
+  uint32_t max_depth;
+  uint64_t seed;
+  uint32_t pixel_base;    /* first accumulator pixel of the job */
+  uint32_t spp;           /* samples per pixel of the whole job */
+  uint32_t fp64;          /* accumulate in f64 */
+  uint32_t guiding;
+  uint32_t pad[2];
+};
+
+/* one job's share of a wavefront pass: `samples` samples of every pixel */
+struct DevItem {
+  uint32_t job;
+  uint32_t path_base;     /* first path slot of the item */
+  uint32_t pixels;        /* W*H of the job */
+  uint32_t samples;       /* samples per pixel in this pass */
+  uint32_t sample_base;   /* absolute index of the item's first sample */
+  uint32_t res_base;      /* first resolve thread of the item (prefix sum of pixels) */
+  uint32_t pad[2];
+};
+
+/* The scene arena: every uploaded scene lives in these arrays with global
+ * indices (child words, record offsets, material / SPD / light indices are
+ * rebased at upload), so one traversal launch serves rays of any scene; a ray
+ * starts at its job's root node. */
 struct DevScene {
   const Bvh4Node *nodes;
   const float4 *prims;
@@ -27,11 +67,9 @@ struct DevScene {
   const DevLight *lights;
   const DevLightNode *lnodes;
   const float *lweights;
-  uint32_t n_lights;
-  uint32_t light_root;
-  int32_t bg_spd;
-  uint32_t has_xforms;
-  DevCamera cam;
+  const DevJob *jobs;
+  uint32_t has_xforms;    /* some scene of the batch has instance chains */
+  uint32_t pad;
 };
 
 /* structure-of-arrays path state, one slot per in-flight sample */
@@ -41,7 +79,7 @@ struct PathBuf {
   float4 *thr;      /* throughput, 4 hero wavelengths */
   float4 *rad;      /* accumulated radiance, 4 hero wavelengths */
   uint64_t *rng;    /* PCG32 state */
-  uint32_t *meta;   /* hero wavelength index (9 bits) | depth << 9 */
+  uint32_t *meta;   /* META_*: hero wavelength index, depth, job */
   float *hit_t;
   uint32_t *hit_p;  /* primitive record offset of the closest hit, ~0u on a miss */
 };
@@ -62,25 +100,28 @@ struct TraceArgs {
   unsigned long long *stats;   /* counting mode: node visits, primitive tests */
   uint32_t *ids;               /* primary mode outputs */
   float *ts;
-  uint32_t lds_nodes;          /* LDS variant: nodes [0, lds_nodes) are cached in LDS */
-  uint32_t lds_units;          /* LDS variant: primitive float4 units [0, lds_units) are cached */
+  uint32_t lds_node_base;      /* LDS variant: nodes [base, base + lds_nodes) are cached in LDS */
+  uint32_t lds_nodes;
+  uint32_t lds_unit_base;      /* LDS variant: primitive float4 units [base, base + lds_units) are cached */
+  uint32_t lds_units;
 };
 
 struct GenParams {
-  uint32_t width, height, pixels, paths;  /* paths = pixels * samples in this pass */
-  uint32_t sample_base;
-  uint32_t pad;
-  uint64_t seed;
+  const DevItem *items;   /* the pass' items, ordered by path_base */
+  uint32_t n_items;
+  uint32_t paths;         /* total paths of the pass */
 };
 
 struct ResolveParams {
-  uint32_t pixels;
-  uint32_t per_pixel;  /* samples of this pass */
-  uint32_t fp64;
-  uint32_t pad;
-  float *acc32;        /* 3 per pixel */
-  double *acc64;       /* 3 per pixel */
-  double *acc_sq;      /* 3 per pixel */
+  const DevItem *items;   /* the pass' items (resolve) */
+  uint32_t n_items;
+  uint32_t threads;       /* resolve: sum of the items' pixels */
+  const DevJob *jobs;     /* finalize / zero: all jobs */
+  uint32_t n_jobs;
+  uint32_t pixels;        /* finalize / zero: sum of all jobs' pixels */
+  float *acc32;           /* 3 per pixel */
+  double *acc64;          /* 3 per pixel */
+  double *acc_sq;         /* 3 per pixel */
 };
 
 /* stats: 4 x u64 = node visits, primitive tests, rays traced, spare */
@@ -111,9 +152,9 @@ void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const 
 void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                     int variant);
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
-                  uint32_t *counts, uint32_t out_slot, uint32_t max_depth, uint32_t max_paths, uint32_t grid_cap);
+                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap);
 void launch_resolve(hipStream_t st, const PathBuf &pb, const ResolveParams &rp, const float4 *cie);
-void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float *out_sq, uint32_t spp);
+void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float *out_sq);
 void launch_zero_accum(hipStream_t st, const ResolveParams &rp);
 
 }  // namespace clv

@@ -280,7 +280,7 @@ __device__ __forceinline__ void lane_start(Lane &L, const PathBuf &pb, uint32_t 
    * bvh/hitable_trait.rs:43-46); without this the NaN-blind slab test below
    * would accept every box and walk the whole tree */
   const float chk = L.o.x + L.o.y + L.o.z + L.d.x + L.d.y + L.d.z;
-  L.word = (chk == chk) ? 0u : CHILD_EMPTY; /* root node */
+  L.word = (chk == chk) ? as_u(o4.w) : CHILD_EMPTY; /* the ray's root node: its job's scene in the arena */
   L.sp = 0;
   L.path = path;
   L.live = true;
@@ -374,15 +374,32 @@ __global__ void k_set_count(uint32_t *counts, uint32_t slot, uint32_t value) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counts[slot] = value;
 }
 
-/* camera sample of sampler/random.rs:20-32 and Camera::get_ray (camera.rs:103-126) */
+/* index of the last entry of a table ordered by `key` whose key is <= v */
+template <typename T>
+__device__ __forceinline__ uint32_t find_entry(const T *tab, uint32_t n, uint32_t v, uint32_t T::*key) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1u) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (tab[mid].*key <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+/* camera sample of sampler/random.rs:20-32 and Camera::get_ray (camera.rs:103-126);
+ * path slot i belongs to the pass item whose range holds it */
 __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenParams gp, uint32_t *qout) {
   uint32_t n = gp.paths;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    uint32_t j = i / gp.pixels, o = i - j * gp.pixels;
-    uint32_t r = o / gp.width, x = o - r * gp.width;
-    uint32_t y = gp.height - 1u - r;
-    uint32_t sample = gp.sample_base + j;
-    uint64_t s = pcg_for_path(gp.seed, x + y * gp.width, sample);
+    const DevItem it = gp.items[find_entry(gp.items, gp.n_items, i, &DevItem::path_base)];
+    const DevJob &jb = sc.jobs[it.job];
+    const uint32_t W = jb.width, H = jb.height;
+    uint32_t k = i - it.path_base;
+    uint32_t j = k / it.pixels, o = k - j * it.pixels;
+    uint32_t r = o / W, x = o - r * W;
+    uint32_t y = H - 1u - r;
+    uint32_t sample = it.sample_base + j;
+    uint64_t s = pcg_for_path(jb.seed, x + y * W, sample);
     float ox = rnd(s), oy = rnd(s);
     float lx, ly;
     for (;;) {
@@ -392,9 +409,9 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
     }
     (void)rnd(s); /* time: unused by the camera, drawn to keep the stream order */
     uint32_t wl = (uint32_t)(((uint64_t)pcg_next(s) * 400ull) >> 32);
-    float u = ((float)x + ox) / (float)gp.width;
-    float v = ((float)y + oy) / (float)gp.height;
-    const DevCamera &c = sc.cam;
+    float u = ((float)x + ox) / (float)W;
+    float v = ((float)y + oy) / (float)H;
+    const DevCamera &c = jb.cam;
     lx *= c.lens_radius;
     ly *= c.lens_radius;
     V cu = mk(c.u[0], c.u[1], c.u[2]), cv = mk(c.v[0], c.v[1], c.v[2]);
@@ -404,12 +421,12 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
             v * mk(c.vertical[0], c.vertical[1], c.vertical[2]) - org - off;
     dir = unit(dir);
     org = org + off;
-    pb.ray_o[i] = make_float4(org.x, org.y, org.z, 0.0f);
+    pb.ray_o[i] = make_float4(org.x, org.y, org.z, __uint_as_float(jb.root));
     pb.ray_d[i] = make_float4(dir.x, dir.y, dir.z, 0.0f);
     pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     pb.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pb.rng[i] = s;
-    pb.meta[i] = wl; /* depth 0 */
+    pb.meta[i] = wl | (it.job << META_JOB_SHIFT); /* depth 0 */
     qout[i] = i;
   }
 }
@@ -494,16 +511,16 @@ __device__ __forceinline__ void lprim_at(Lane &L, const DevScene &sc, const floa
 template <bool COUNT, int MODE>
 __global__ __launch_bounds__(LT_BLOCK) void k_trace_lds(DevScene sc, PathBuf pb, TraceArgs ta) {
   extern __shared__ float4 smem[];
-  const uint32_t nn = ta.lds_nodes, nu = ta.lds_units;
+  const uint32_t nn = ta.lds_nodes, nu = ta.lds_units, nb = ta.lds_node_base, ub = ta.lds_unit_base;
   float4 *cnodes = smem;
   float4 *cprims = smem + nn * 8u;
   LStack S;
   S.lst = reinterpret_cast<uint32_t *>(cprims + nu) + threadIdx.x;
   S.gst = ta.spill + (size_t)(blockIdx.x * LT_BLOCK + threadIdx.x) * LT_STACK_GLOBAL;
   {
-    const float4 *gn = reinterpret_cast<const float4 *>(sc.nodes);
+    const float4 *gn = reinterpret_cast<const float4 *>(sc.nodes + nb);
     for (uint32_t k = threadIdx.x; k < nn * 8u; k += LT_BLOCK) cnodes[k] = gn[k];
-    for (uint32_t k = threadIdx.x; k < nu; k += LT_BLOCK) cprims[k] = sc.prims[k];
+    for (uint32_t k = threadIdx.x; k < nu; k += LT_BLOCK) cprims[k] = sc.prims[ub + k];
   }
   __syncthreads();
   __shared__ uint32_t agg[MC_COUNT * (LT_BLOCK / 64) + MC_COUNT];
@@ -521,7 +538,8 @@ __global__ __launch_bounds__(LT_BLOCK) void k_trace_lds(DevScene sc, PathBuf pb,
     else L.word = CHILD_EMPTY;
     while (L.word != CHILD_EMPTY) {
       if (!(L.word & CHILD_LEAF)) {
-        if (L.word < nn) lnode_at(L, cnodes + L.word * 8u, S);
+        const uint32_t rel = L.word - nb; /* wraps above nn for nodes below the window */
+        if (rel < nn) lnode_at(L, cnodes + rel * 8u, S);
         else lnode_at(L, reinterpret_cast<const float4 *>(sc.nodes + L.word), S);
         if (COUNT) nvisit++;
       } else {
@@ -529,7 +547,7 @@ __global__ __launch_bounds__(LT_BLOCK) void k_trace_lds(DevScene sc, PathBuf pb,
         uint32_t off = L.word & LEAF_OFFSET_MASK;
         for (uint32_t k = 0; k < cnt; k++) {
           uint32_t units;
-          if (off + PREC_QUAD <= nu) lprim_at(L, sc, cprims + off, off, any_xform, units);
+          if (off >= ub && off + PREC_QUAD <= ub + nu) lprim_at(L, sc, cprims + (off - ub), off, any_xform, units);
           else lprim_at(L, sc, sc.prims + off, off, any_xform, units);
           off += units;
         }
@@ -898,8 +916,8 @@ __device__ V light_sample(const DevLight &L, V o, uint64_t &s) {
 /* HitablePDF over the MIS tree: sampling descends with a fair coin wherever
  * both children exist (bvh/hitable_trait.rs:113-127); the pdf is the
  * 2^-depth weighted sum over leaves, i.e. the nested midpoints of :89-111. */
-__device__ V tree_sample(const DevScene &sc, V o, uint64_t &s) {
-  uint32_t w = sc.light_root;
+__device__ V tree_sample(const DevScene &sc, const DevJob &jb, V o, uint64_t &s) {
+  uint32_t w = jb.light_root;
   while (!(w & LT_LEAF)) {
     const DevLightNode &nd = sc.lnodes[w];
     if (nd.child[1] == LT_EMPTY) w = nd.child[0];
@@ -908,9 +926,9 @@ __device__ V tree_sample(const DevScene &sc, V o, uint64_t &s) {
   }
   return light_sample(sc.lights[w & ~LT_LEAF], o, s);
 }
-__device__ float tree_pdf(const DevScene &sc, V o, V d) {
+__device__ float tree_pdf(const DevScene &sc, const DevJob &jb, V o, V d) {
   float p = 0.0f;
-  for (uint32_t l = 0; l < sc.n_lights; l++) {
+  for (uint32_t l = jb.light_base; l < jb.light_base + jb.n_lights; l++) {
     float w = sc.lweights[l];
     if (w > 0.0f) p += w * light_pdf(sc.lights[l], o, d);
   }
@@ -924,15 +942,16 @@ __device__ __forceinline__ void rotate_wl(uint32_t hero_idx, int wl[4]) {
 
 /* One shading step of one path; returns true if the path continues. */
 template <uint32_t CLS>
-__device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint32_t i, uint32_t max_depth) {
+__device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint32_t i) {
   uint32_t meta = pb.meta[i];
-  uint32_t hero_idx = meta & 0x1FFu;
-  uint32_t depth = meta >> 9;
+  uint32_t hero_idx = meta & META_HERO_MASK;
+  uint32_t depth = (meta >> META_DEPTH_SHIFT) & META_DEPTH_MASK;
+  const DevJob &jb = sc.jobs[meta >> META_JOB_SHIFT];
   int wl[4];
   rotate_wl(hero_idx, wl);
   float4 T = pb.thr[i];
   if (CLS == MC_MISS) {
-    const float *bg = sc.spd + sc.bg_spd;
+    const float *bg = sc.spd + jb.bg_spd;
     float4 L = pb.rad[i];
     L.x += T.x * bg[wl[0] - CLV_LAMBDA_MIN];
     L.y += T.y * bg[wl[1] - CLV_LAMBDA_MIN];
@@ -978,7 +997,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     Onb b = onb_from_w(h.n);
     V gen;
     if (coin(s)) {
-      gen = tree_sample(sc, h.p, s);
+      gen = tree_sample(sc, jb, h.p, s);
     } else {
       float r1 = rnd(s), r2 = rnd(s);
       float z = sqrtf(1.0f - r2);
@@ -986,7 +1005,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
       gen = onb_local(b, mk(cosf(phi) * sqrtf(r2), sinf(phi) * sqrtf(r2), z));
     }
     nd = unit(gen);
-    float lp = tree_pdf(sc, h.p, nd);
+    float lp = tree_pdf(sc, jb, h.p, nd);
     V ndn = unit(nd);
     float cs = dot(ndn, b.w);
     float cp = cs <= 0.0f ? 0.0f : cs / PI_F;
@@ -1029,16 +1048,16 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     T.w *= att[3];
   }
   pb.rng[i] = s;
-  if (depth + 1u > max_depth) return false; /* trace() at depth > max_depth returns 0 */
+  if (depth + 1u > jb.max_depth) return false; /* trace() at depth > max_depth returns 0 */
   pb.thr[i] = T;
-  pb.ray_o[i] = make_float4(h.p.x, h.p.y, h.p.z, 0.0f);
+  pb.ray_o[i] = make_float4(h.p.x, h.p.y, h.p.z, o4.w); /* .w: root node of the path's scene */
   pb.ray_d[i] = make_float4(nd.x, nd.y, nd.z, 0.0f);
-  pb.meta[i] = hero_idx | ((depth + 1u) << 9);
+  pb.meta[i] = (meta & ~(META_DEPTH_MASK << META_DEPTH_SHIFT)) | ((depth + 1u) << META_DEPTH_SHIFT);
   return true;
 }
 
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQueues sq, uint32_t *qout,
-                                                uint32_t *counts, uint32_t out_slot, uint32_t max_depth) {
+                                                uint32_t *counts, uint32_t out_slot) {
   __shared__ uint32_t agg[1 * 4 + 1];
   uint32_t c[MC_COUNT], chunks[MC_COUNT + 1];
   chunks[0] = 0;
@@ -1065,11 +1084,11 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
       if (idx < c[cls]) {
         path = sq.q[cls][idx];
         switch (cls) {
-          case MC_MISS: cont = shade_path<MC_MISS>(sc, pb, path, max_depth); break;
-          case MC_LIGHT: cont = shade_path<MC_LIGHT>(sc, pb, path, max_depth); break;
-          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN>(sc, pb, path, max_depth); break;
-          case MC_METAL: cont = shade_path<MC_METAL>(sc, pb, path, max_depth); break;
-          default: cont = shade_path<MC_GLASS>(sc, pb, path, max_depth); break;
+          case MC_MISS: cont = shade_path<MC_MISS>(sc, pb, path); break;
+          case MC_LIGHT: cont = shade_path<MC_LIGHT>(sc, pb, path); break;
+          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN>(sc, pb, path); break;
+          case MC_METAL: cont = shade_path<MC_METAL>(sc, pb, path); break;
+          default: cont = shade_path<MC_GLASS>(sc, pb, path); break;
         }
       }
     }
@@ -1077,20 +1096,26 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
   }
 }
 
-/* per-sample estimator (draw_cpu.rs:130-142) and per-pixel sums in sample order */
+/* per-sample estimator (draw_cpu.rs:130-142) and per-pixel sums in sample
+ * order; one thread per (pass item, pixel) */
 __global__ __launch_bounds__(256) void k_resolve(PathBuf pb, ResolveParams rp, const float4 *cie) {
-  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < rp.pixels; o += gridDim.x * blockDim.x) {
+  const DevJob *jobs = rp.jobs;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < rp.threads; g += gridDim.x * blockDim.x) {
+    const DevItem it = rp.items[find_entry(rp.items, rp.n_items, g, &DevItem::res_base)];
+    const uint32_t o = g - it.res_base;
+    const bool fp64 = jobs[it.job].fp64 != 0;
+    const uint32_t a = 3u * (jobs[it.job].pixel_base + o);
     float a32[3];
     double a64[3], sq[3];
     for (int k = 0; k < 3; k++) {
-      sq[k] = rp.acc_sq[3 * o + k];
-      if (rp.fp64) a64[k] = rp.acc64[3 * o + k];
-      else a32[k] = rp.acc32[3 * o + k];
+      sq[k] = rp.acc_sq[a + k];
+      if (fp64) a64[k] = rp.acc64[a + k];
+      else a32[k] = rp.acc32[a + k];
     }
-    for (uint32_t j = 0; j < rp.per_pixel; j++) {
-      uint32_t i = j * rp.pixels + o;
+    for (uint32_t j = 0; j < it.samples; j++) {
+      uint32_t i = it.path_base + j * it.pixels + o;
       float4 L = pb.rad[i];
-      uint32_t hero = pb.meta[i] & 0x1FFu;
+      uint32_t hero = pb.meta[i] & META_HERO_MASK;
       float c[3] = {0.0f, 0.0f, 0.0f};
       if (L.y == 0.0f && L.z == 0.0f && L.w == 0.0f) {
         float4 x = cie[hero];
@@ -1111,23 +1136,25 @@ __global__ __launch_bounds__(256) void k_resolve(PathBuf pb, ResolveParams rp, c
         }
       }
       for (int k = 0; k < 3; k++) {
-        if (rp.fp64) a64[k] += (double)c[k];
+        if (fp64) a64[k] += (double)c[k];
         else a32[k] += c[k];
         sq[k] += (double)c[k] * (double)c[k];
       }
     }
     for (int k = 0; k < 3; k++) {
-      rp.acc_sq[3 * o + k] = sq[k];
-      if (rp.fp64) rp.acc64[3 * o + k] = a64[k];
-      else rp.acc32[3 * o + k] = a32[k];
+      rp.acc_sq[a + k] = sq[k];
+      if (fp64) rp.acc64[a + k] = a64[k];
+      else rp.acc32[a + k] = a32[k];
     }
   }
 }
 
-__global__ void k_finalize(ResolveParams rp, float *out, float *out_sq, uint32_t spp) {
+/* per-pixel means of every job (pixels of job j start at jobs[j].pixel_base) */
+__global__ void k_finalize(ResolveParams rp, float *out, float *out_sq) {
   for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < rp.pixels * 3u; o += gridDim.x * blockDim.x) {
-    out[o] = rp.fp64 ? (float)(rp.acc64[o] / (double)spp) : rp.acc32[o] / (float)spp;
-    if (out_sq) out_sq[o] = (float)(rp.acc_sq[o] / (double)spp);
+    const DevJob &jb = rp.jobs[find_entry(rp.jobs, rp.n_jobs, o / 3u, &DevJob::pixel_base)];
+    out[o] = jb.fp64 ? (float)(rp.acc64[o] / (double)jb.spp) : rp.acc32[o] / (float)jb.spp;
+    if (out_sq) out_sq[o] = (float)(rp.acc_sq[o] / (double)jb.spp);
   }
 }
 
@@ -1227,15 +1254,15 @@ void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const
     hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
 }
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
-                  uint32_t *counts, uint32_t out_slot, uint32_t max_depth, uint32_t max_paths, uint32_t grid_cap) {
+                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap) {
   uint32_t g = grid_for(max_paths + 64u * MC_COUNT, 256, grid_cap);
-  hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, max_depth);
+  hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot);
 }
 void launch_resolve(hipStream_t st, const PathBuf &pb, const ResolveParams &rp, const float4 *cie) {
-  hipLaunchKernelGGL(k_resolve, dim3(grid_for(rp.pixels, 256, 65535)), dim3(256), 0, st, pb, rp, cie);
+  hipLaunchKernelGGL(k_resolve, dim3(grid_for(rp.threads, 256, 65535)), dim3(256), 0, st, pb, rp, cie);
 }
-void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float *out_sq, uint32_t spp) {
-  hipLaunchKernelGGL(k_finalize, dim3(grid_for(rp.pixels * 3u, 256, 65535)), dim3(256), 0, st, rp, out, out_sq, spp);
+void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float *out_sq) {
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(rp.pixels * 3u, 256, 65535)), dim3(256), 0, st, rp, out, out_sq);
 }
 void launch_zero_accum(hipStream_t st, const ResolveParams &rp) {
   hipLaunchKernelGGL(k_zero_accum, dim3(grid_for(rp.pixels * 3u, 256, 65535)), dim3(256), 0, st, rp);

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CLV_ABI_VERSION 1u
+#define CLV_ABI_VERSION 2u
 
 typedef enum {
   CLV_OK = 0,
@@ -219,6 +219,29 @@ clv_status clv_scene_validate(const clv_scene_desc *desc, clv_scene_info *info);
  * Blocking. */
 clv_status clv_render(clv_ctx *ctx, uint32_t scene_id, const clv_render_opts *opts,
                       float *xyz_out, float *xyz_sq_out);
+
+/* One job of a batched render: one view of one uploaded scene.  No reference
+ * counterpart -- the reference renders one scene per process run
+ * (clovers-cli/src/render.rs:137-176); this is how a caller renders several
+ * views or scenes in one go.  Jobs of different scenes, sizes, sample counts
+ * and depths share the wavefront passes of one launch sequence. */
+typedef struct {
+  uint32_t scene_id;
+  uint32_t use_camera;       /* 1: `camera` replaces the scene's camera for this job */
+  clv_camera_init camera;
+  uint32_t _pad;
+  clv_render_opts opts;      /* per job: size, spp, depth, seed, sample_offset, fp64_accum, guiding;
+                                flags are OR-ed over the batch */
+  float *xyz_out;            /* W*H*3 floats, as clv_render */
+  float *xyz_sq_out;         /* nullable, as clv_render */
+} clv_job;                   /* 128 bytes */
+
+/* Render every job (at most 8192, at most 2^28 pixels in all).  Each job's
+ * output equals what clv_render gives for it alone: paths are keyed by
+ * (job seed, pixel, sample), so batching does not change any path.  Blocking.
+ * clv_counters then covers the whole batch (BVH fields: the job with the most
+ * samples). */
+clv_status clv_render_batch(clv_ctx *ctx, const clv_job *jobs, uint32_t n_jobs);
 
 /* Primary-ray closest hits for sample `opts->sample_offset` of every pixel,
  * same camera and ray seeds as clv_render (the parity probe of the hit path).

@@ -36,9 +36,10 @@ def test_struct_layouts_match_the_header():
 #include <stddef.h>
 #include "clovers_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(clv_object), sizeof(clv_material), sizeof(clv_xform),
-         sizeof(clv_camera_init), sizeof(clv_scene_desc), sizeof(clv_render_opts), sizeof(clv_counters),
-         sizeof(clv_scene_info), offsetof(clv_scene_desc, camera));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(clv_object), sizeof(clv_material),
+         sizeof(clv_xform), sizeof(clv_camera_init), sizeof(clv_scene_desc), sizeof(clv_render_opts),
+         sizeof(clv_counters), sizeof(clv_scene_info), offsetof(clv_scene_desc, camera), sizeof(clv_job),
+         offsetof(clv_job, opts), offsetof(clv_job, xyz_out));
   return 0;
 }
 '''
@@ -50,7 +51,8 @@ int main(void) {
         got = [int(v) for v in subprocess.check_output([exe]).split()]
     want = [abi.OBJECT_DTYPE.itemsize, abi.MATERIAL_DTYPE.itemsize, abi.XFORM_DTYPE.itemsize,
             C.sizeof(abi.CameraInit), C.sizeof(abi.SceneDesc), C.sizeof(abi.RenderOpts), C.sizeof(abi.Counters),
-            C.sizeof(abi.SceneInfo), abi.SceneDesc.camera.offset]
+            C.sizeof(abi.SceneInfo), abi.SceneDesc.camera.offset, C.sizeof(abi.Job), abi.Job.opts.offset,
+            abi.Job.xyz_out.offset]
     assert got == want
 
 
@@ -59,6 +61,7 @@ def test_no_device_calls_fail_cleanly_without_gpu():
     lib = clovers_amd.load_library()
     assert lib.clv_render(None, 1, None, None, None) == abi.EINVAL
     assert lib.clv_primary_hits(None, 1, None, None, None) == abi.EINVAL
+    assert lib.clv_render_batch(None, None, 0) == abi.EINVAL
     assert lib.clv_counters_get(None, None) == abi.EINVAL
     assert lib.clv_scene_validate(None, None) == abi.EINVAL
     out = (C.c_float * 3)()

@@ -1,0 +1,99 @@
+"""Batched renders (clv_render_batch) and the shared scene arena.
+
+Paths are keyed by (job seed, pixel, sample) and every pixel sums its samples
+in sample order, so a job rendered inside a batch -- whatever the pass split,
+the other jobs or the scene placement in the arena -- must give exactly the
+bytes clv_render gives for it alone.
+"""
+import numpy as np
+import pytest
+
+import clovers_amd
+from clovers_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def tiny_mesh(view=0):
+    return synth.mesh(freq=6, view=view)  # 720 triangles
+
+
+def test_batch_jobs_equal_single_renders():
+    with clovers_amd.Renderer(0) as r:
+        a = r.upload(synth.cornell())
+        b = r.upload(synth.spheres())
+        c = r.upload(synth.prism())
+        jobs = [
+            (a, abi.render_opts(96, 64, 5, 8, seed=3)),
+            (b, abi.render_opts(160, 90, 3, 16, seed=4)),
+            (c, abi.render_opts(64, 64, 7, 24, seed=5, fp64_accum=1)),
+            (a, abi.render_opts(33, 17, 2, 3, seed=6, sample_offset=11)),
+        ]
+        batch = r.render_batch(jobs, want_sq=True)
+        c_all = r.counters()
+        assert c_all["samples"] == sum(o.width * o.height * o.samples for _, o in jobs)
+        for (sid, o), (img, sq) in zip(jobs, batch):
+            ref, ref_sq = r.render(sid, o, want_sq=True)
+            assert img.shape == (o.height, o.width, 3)
+            assert np.array_equal(img, ref), (sid, np.abs(img - ref).max())
+            assert np.array_equal(sq, ref_sq)
+
+
+def test_batch_many_passes_equal_single_renders(monkeypatch):
+    """A small pass size forces every job across several passes and splits a
+    pass between jobs."""
+    monkeypatch.setenv("CLV_TARGET_PATHS", "65536")
+    with clovers_amd.Renderer(0) as r:
+        a = r.upload(synth.cornell())
+        b = r.upload(tiny_mesh())
+        jobs = [(a, abi.render_opts(128, 128, 9, 8, seed=1)),
+                (b, abi.render_opts(200, 120, 6, 12, seed=2)),
+                (a, abi.render_opts(64, 48, 13, 5, seed=3))]
+        batch = r.render_batch(jobs)
+        assert r.counters()["passes"] > 3
+        for (sid, o), (img, _) in zip(jobs, batch):
+            ref, _ = r.render(sid, o)
+            assert np.array_equal(img, ref)
+
+
+def test_camera_override_equals_scene_camera():
+    with clovers_amd.Renderer(0) as r:
+        s0 = r.upload(tiny_mesh(view=0))
+        s1 = r.upload(tiny_mesh(view=1))
+        o = abi.render_opts(80, 45, 4, 12, seed=9)
+        lf = synth.MESH_VIEWS[1]
+        cam = abi.camera_init(lf, (0.0, 1.0, 0.0), vertical_fov=40.0, aperture=0.0, focus_distance=10.0,
+                              time_0=0.0, time_1=1.0)
+        (img, _), = r.render_batch([(s0, o, cam)])
+        ref, _ = r.render(s1, o)
+        assert np.array_equal(img, ref)
+
+
+def test_arena_reuse_after_free():
+    o = abi.render_opts(64, 64, 2, 8, seed=2)
+    with clovers_amd.Renderer(0) as fresh:
+        want_c = fresh.render(fresh.upload(synth.cornell()), o)[0]
+        want_p = fresh.render(fresh.upload(synth.prism()), o)[0]
+    with clovers_amd.Renderer(0) as r:
+        a = r.upload(synth.cornell())
+        b = r.upload(synth.spheres())
+        r.free(b)                      # last in the arena: its range is reused
+        c = r.upload(synth.prism())
+        assert np.array_equal(r.render(a, o)[0], want_c)
+        assert np.array_equal(r.render(c, o)[0], want_p)
+        r.free(a)                      # a hole below c
+        assert np.array_equal(r.render(c, o)[0], want_p)
+        ids, _ = r.primary_hits(c, abi.render_opts(64, 64, 1, 8, seed=2))
+        assert (ids != abi.MISS).any()
+
+
+def test_batch_errors():
+    with clovers_amd.Renderer(0) as r:
+        a = r.upload(synth.cornell())
+        with pytest.raises(clovers_amd.ClvError) as e:
+            r.render_batch([(a, abi.render_opts(8, 8, 1, 4)), (999, abi.render_opts(8, 8, 1, 4))])
+        assert e.value.status == abi.EINVAL
+        with pytest.raises(clovers_amd.ClvError) as e:
+            r.render_batch([(a, abi.render_opts(8, 8, 0, 4))])
+        assert e.value.status == abi.EINVAL
+        assert r.lib.clv_render_batch(r.h, None, 0) == abi.EINVAL
