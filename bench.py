@@ -25,8 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOADS = {
-    "spheres": dict(scene="spheres", width=1920, height=1080, samples=64, max_depth=16),
-    "cornell": dict(scene="cornell", width=256, height=256, samples=16, max_depth=8),
+    "spheres": dict(scene="spheres", width=1920, height=1080, samples=64, max_depth=16, views=1, count_spp=64),
+    "cornell": dict(scene="cornell", width=256, height=256, samples=16, max_depth=8, views=1, count_spp=16),
+    # 1,003,520-triangle noise-displaced icosphere, 4 camera views in one batch
+    "mesh": dict(scene="mesh", width=1920, height=1080, samples=256, max_depth=12, views=4, count_spp=8),
 }
 HBM_PEAK_GBS = 8000.0
 RAY_STREAM_BYTES = 48  # per traced ray: queue index in/out (8), origin+direction (32), hit t+record (8)
@@ -34,6 +36,18 @@ RAY_STREAM_BYTES = 48  # per traced ray: queue index in/out (8), origin+directio
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def view_jobs(sid, w, opts_fn):
+    """the workload's jobs: view 0 with the scene camera, views 1.. with the
+    other camera azimuths of the mesh scene"""
+    from clovers_amd import abi, synth
+    jobs = [(sid, opts_fn())]
+    for v in range(1, w["views"]):
+        cam = abi.camera_init(synth.MESH_VIEWS[v], (0.0, 1.0, 0.0), vertical_fov=40.0, aperture=0.0,
+                              focus_distance=10.0)
+        jobs.append((sid, opts_fn(), cam))
+    return jobs
 
 
 def cpu_baseline(scene, w, seconds):
@@ -50,15 +64,21 @@ def cpu_baseline(scene, w, seconds):
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     sc = po.OracleScene(scene, library=lib)
     opts = abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1)
-    # calibrate on 2 rows, then time a slice of about `seconds`
-    t0 = time.perf_counter()
-    sc.render(opts, threads=threads, rows=(0, 2))
-    per_row = max((time.perf_counter() - t0) / 2, 1e-4)
-    rows = int(max(2, min(w["height"], seconds / per_row)))
-    r0 = (w["height"] - rows) // 2
-    t0 = time.perf_counter()
-    sc.render(opts, threads=threads, rows=(r0, r0 + rows))
-    dt = time.perf_counter() - t0
+    # a central band growing outwards in slices of doubling height until about
+    # `seconds` of CPU work are timed
+    H = w["height"]
+    lo = hi = H // 2
+    dt, chunk = 0.0, 2
+    while dt < seconds and (lo > 0 or hi < H):
+        a = min(max(chunk // 2, 1), lo)
+        b = min(chunk - a, H - hi)
+        for r0, r1 in ((lo - a, lo), (hi, hi + b)):
+            if r1 > r0:
+                t0 = time.perf_counter()
+                sc.render(opts, threads=threads, rows=(r0, r1))
+                dt += time.perf_counter() - t0
+        lo, hi, chunk = lo - a, hi + b, chunk * 2
+    rows = hi - lo
     samples = rows * w["width"] * w["samples"]
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": "%d central rows of %dx%d at %d spp, depth %d (%d samples, %.1f s); CPU oracle = C++ "
@@ -74,6 +94,9 @@ def main():
     ap.add_argument("--workload", default="spheres", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spp", type=int, default=0,
+                    help="override the workload's samples per pixel (profiling runs: per-launch figures do not "
+                         "depend on it; the JSON line then names the reduced workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="before rendering, copy 1 GiB device-to-device 3 times (a plain copy kernel of known "
                          "size, so profiler byte counters can be scaled)")
@@ -98,59 +121,73 @@ def main():
         log("calibration copy: 3 x %d bytes" % (a.numel() * 4))
         del a, b
 
-    w = WORKLOADS[args.workload]
+    w = dict(WORKLOADS[args.workload])
+    if args.spp:
+        w["samples"] = args.spp
+        w["count_spp"] = min(w["count_spp"], args.spp)
     scene = getattr(synth, w["scene"])()
     log("workload", args.workload, w)
     r = clovers_amd.Renderer(0)
     t0 = time.perf_counter()
     sid = r.upload(scene)
     log("scene upload %.1f ms (host build %.1f ms)" % ((time.perf_counter() - t0) * 1e3, r.counters()["build_ms"]))
-    opts = abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1)
+
+    def opts_fn(spp=w["samples"], flags=0):
+        return lambda: abi.render_opts(w["width"], w["height"], spp, w["max_depth"], seed=1, flags=flags)
+
+    jobs = view_jobs(sid, w, opts_fn())
     for i in range(args.warmup):
-        r.render(sid, opts)
+        r.render_batch(jobs)
         log("warmup %d done" % i)
     gpu_ms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        img, _ = r.render(sid, opts)
+        outs = r.render_batch(jobs)
         gpu_ms.append(r.counters()["render_ms"])
         log("step %d: %.1f ms device" % (i, gpu_ms[-1]))
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
-    samples = w["width"] * w["height"] * w["samples"]
+    samples = w["width"] * w["height"] * w["samples"] * w["views"]
     value = samples / (ms_per_step / 1e3) / 1e6
-    assert np.all(np.isfinite(img))
+    # The reference's estimator lets a NaN hero-only sample through (draw_cpu.rs:131-133; thin-film and
+    # refraction take sqrt(1 - c^2) unclamped, thin_film.rs:45-48, materials.rs:194-200), so a head-on
+    # hit on the thin-film mesh can leave a non-finite pixel there too; count them.
+    nonfinite = int(sum((~np.isfinite(img)).any(axis=-1).sum() for img, _ in outs))
+    log("non-finite pixels: %d" % nonfinite)
 
     # dominant kernel: traversal.  Per-launch durations from HIP events ...
-    r.render(sid, abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1,
-                                  flags=abi.FLAG_TIME_KERNELS))
+    r.render_batch(view_jobs(sid, w, opts_fn(flags=abi.FLAG_TIME_KERNELS)))
     ct = r.counters()
     avg_launch_ms = ct["extend_ms"] / ct["extend_launches"]
-    log("traversal: %d launches, %.2f ms total, %.4f ms mean" % (ct["extend_launches"], ct["extend_ms"], avg_launch_ms))
-    # ... and algorithmic bytes per launch from the counting traversal
-    r.render(sid, abi.render_opts(w["width"], w["height"], w["samples"], w["max_depth"], seed=1,
-                                  flags=abi.FLAG_COUNT))
+    rays_per_launch = ct["rays"] / ct["extend_launches"]
+    log("traversal: %d launches, %.2f ms total, %.4f ms mean, %.0f rays per launch"
+        % (ct["extend_launches"], ct["extend_ms"], avg_launch_ms, rays_per_launch))
+    # ... and algorithmic bytes per ray from the counting traversal (the same
+    # paths; for the mesh a prefix of the samples: the per-ray means converge)
+    r.render_batch(view_jobs(sid, w, opts_fn(spp=w["count_spp"], flags=abi.FLAG_COUNT)))
     cc = r.counters()
-    alg_bytes = (cc["node_visits"] * cc["node_bytes"] + cc["prim_tests"] * cc["prim_bytes"]
-                 + cc["rays"] * RAY_STREAM_BYTES)
-    per_launch = alg_bytes / cc["extend_launches"]
+    per_ray = (cc["node_visits"] * cc["node_bytes"] + cc["prim_tests"] * cc["prim_bytes"]) / cc["rays"] \
+        + RAY_STREAM_BYTES
+    per_launch = per_ray * rays_per_launch
     achieved = per_launch / (avg_launch_ms * 1e-3) / 1e9
-    log("rays %d (%.2f per sample), nodes/ray %.1f, prims/ray %.1f"
-        % (cc["rays"], cc["rays"] / samples, cc["node_visits"] / cc["rays"], cc["prim_tests"] / cc["rays"]))
+    log("rays %d (%.2f per sample), nodes/ray %.2f, prims/ray %.2f, %.1f B/ray"
+        % (ct["rays"], ct["rays"] / samples, cc["node_visits"] / cc["rays"], cc["prim_tests"] / cc["rays"], per_ray))
+    lds_note = ("for the sphere field the BVH (20 KB) and spheres (16 KB) are served from LDS, so HBM traffic per "
+                "launch is far below the algorithmic bytes and the kernel is issue/latency bound (VALUBusy, "
+                "VALUUtilization in profiles/)") if args.workload != "mesh" else (
+                "the mesh BVH (40 MB of nodes) and triangles (64 MB) exceed the LDS window and L2; node and "
+                "triangle fetches come from HBM / the MALL (FETCH_SIZE in profiles/)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_extend (4-wide BVH traversal + primitive tests)",
+                "kernel": "k_trace_lds (4-wide BVH traversal + primitive tests)",
                 "bytes_per_launch": int(per_launch), "mean_launch_ms": round(avg_launch_ms, 4),
                 "per_ray": {"node_visits": round(cc["node_visits"] / cc["rays"], 3),
                             "prim_tests": round(cc["prim_tests"] / cc["rays"], 3),
                             "node_bytes": cc["node_bytes"], "prim_bytes": cc["prim_bytes"],
                             "stream_bytes": RAY_STREAM_BYTES},
-                "rays_per_sample": round(cc["rays"] / samples, 4),
+                "rays_per_sample": round(ct["rays"] / samples, 4),
                 "note": "algorithmic bytes = counted node visits x 128 B + primitive tests x record bytes + 48 B "
-                        "of ray stream per ray; for the sphere field the BVH (20 KB) and spheres (16 KB) are "
-                        "served from LDS, so HBM traffic per launch is far below the algorithmic bytes "
-                        "(rocprofv3 FETCH_SIZE/WRITE_SIZE in profiles/) and the kernel is issue/latency bound "
-                        "(VALUBusy, VALUUtilization in profiles/)"}
+                        "of ray stream per ray; " + lds_note}
     r.close()
 
     cpu = None
@@ -163,10 +200,12 @@ def main():
         "metric": "Msamples/s at 1 GPU", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "%s %dx%d %dspp depth%d" % (w["scene"], w["width"], w["height"], w["samples"],
-                                                             w["max_depth"]),
+        "config": {"workload": "%s %dx%d %dspp depth%d%s" % (w["scene"], w["width"], w["height"], w["samples"],
+                                                               w["max_depth"],
+                                                               " x%d views" % w["views"] if w["views"] > 1 else ""),
+                   "views": w["views"],
                    "width": w["width"], "height": w["height"], "spp": w["samples"], "max_depth": w["max_depth"],
-                   "samples_per_step": samples, "device_ms_per_step": round(float(np.mean(gpu_ms)), 3),
+                   "samples_per_step": samples, "nonfinite_pixels": nonfinite, "device_ms_per_step": round(float(np.mean(gpu_ms)), 3),
                    "parallelism": "1 GPU, wavefront"},
         "roofline": roofline,
         "cpu_baseline": cpu,
