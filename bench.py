@@ -31,6 +31,7 @@ WORKLOADS = {
     "mesh": dict(scene="mesh", width=1920, height=1080, samples=256, max_depth=12, views=4, count_spp=8),
 }
 HBM_PEAK_GBS = 8000.0
+TRACE_KERNELS = {0: "k_trace_simple", 1: "k_trace", 2: "k_trace_simple (ww)", 3: "k_trace_lds"}
 RAY_STREAM_BYTES = 48  # per traced ray: queue index in/out (8), origin+direction (32), hit t+record (8)
 
 
@@ -94,6 +95,7 @@ def main():
     ap.add_argument("--workload", default="spheres", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--spp", type=int, default=0,
                     help="override the workload's samples per pixel (profiling runs: per-launch figures do not "
                          "depend on it; the JSON line then names the reduced workload)")
@@ -136,6 +138,12 @@ def main():
         return lambda: abi.render_opts(w["width"], w["height"], spp, w["max_depth"], seed=1, flags=flags)
 
     jobs = view_jobs(sid, w, opts_fn())
+    tuning, tune_ms = r.tuning(), 0.0
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = r.autotune(jobs)
+        tune_ms = (time.perf_counter() - t0) * 1e3
+        log("autotune %.0f ms: %s" % (tune_ms, tuning))
     for i in range(args.warmup):
         r.render_batch(jobs)
         log("warmup %d done" % i)
@@ -179,7 +187,7 @@ def main():
                 "triangle fetches come from HBM / the MALL (FETCH_SIZE in profiles/)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_trace_lds (4-wide BVH traversal + primitive tests)",
+                "kernel": "%s (4-wide BVH traversal + primitive tests)" % TRACE_KERNELS[tuning["trace_kernel"]],
                 "bytes_per_launch": int(per_launch), "mean_launch_ms": round(avg_launch_ms, 4),
                 "per_ray": {"node_visits": round(cc["node_visits"] / cc["rays"], 3),
                             "prim_tests": round(cc["prim_tests"] / cc["rays"], 3),
@@ -206,7 +214,9 @@ def main():
                    "views": w["views"],
                    "width": w["width"], "height": w["height"], "spp": w["samples"], "max_depth": w["max_depth"],
                    "samples_per_step": samples, "nonfinite_pixels": nonfinite, "device_ms_per_step": round(float(np.mean(gpu_ms)), 3),
-                   "parallelism": "1 GPU, wavefront"},
+                   "parallelism": "1 GPU, wavefront",
+                   "tuning": dict(tuning, trace_kernel=abi.TRACE_NAMES[tuning["trace_kernel"]]),
+                   "autotune_ms": round(tune_ms, 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,

@@ -53,6 +53,12 @@ def load_library(path=LIB_PATH):
     lib.clv_render.argtypes = [vp, C.c_uint32, vp, vp, vp]
     lib.clv_render_batch.restype = C.c_int
     lib.clv_render_batch.argtypes = [vp, vp, C.c_uint32]
+    lib.clv_tuning_get.restype = C.c_int
+    lib.clv_tuning_get.argtypes = [vp, vp]
+    lib.clv_tuning_set.restype = C.c_int
+    lib.clv_tuning_set.argtypes = [vp, vp]
+    lib.clv_autotune.restype = C.c_int
+    lib.clv_autotune.argtypes = [vp, vp, C.c_uint32, C.c_uint64, vp]
     lib.clv_primary_hits.restype = C.c_int
     lib.clv_primary_hits.argtypes = [vp, C.c_uint32, vp, vp, vp]
     lib.clv_counters_get.restype = C.c_int
@@ -146,10 +152,7 @@ class Renderer:
                                         sq.ctypes.data if sq is not None else None))
         return out, sq
 
-    def render_batch(self, jobs, want_sq=False):
-        """clv_render_batch.  jobs: list of (scene_id, opts) or (scene_id, opts,
-        camera) with camera an abi.CameraInit replacing the scene's camera.
-        Returns [(xyz[H, W, 3], sq or None)] in job order."""
+    def _jobs(self, jobs, want_sq):
         arr = (abi.Job * len(jobs))()
         outs = []
         for k, jb in enumerate(jobs):
@@ -165,6 +168,32 @@ class Renderer:
             arr[k].xyz_out = out.ctypes.data
             arr[k].xyz_sq_out = sq.ctypes.data if sq is not None else None
             outs.append((out, sq))
+        return arr, outs
+
+    def tuning(self):
+        t = abi.Tuning()
+        self._check(self.lib.clv_tuning_get(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in abi.Tuning._fields_ if k != "_pad"}
+
+    def set_tuning(self, **kw):
+        t = abi.Tuning()
+        self._check(self.lib.clv_tuning_get(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._check(self.lib.clv_tuning_set(self.h, C.byref(t)))
+
+    def autotune(self, jobs, trial_samples=0):
+        """clv_autotune over the batch `jobs` (as render_batch); returns the kept tuning."""
+        arr, _ = self._jobs(jobs, False)
+        t = abi.Tuning()
+        self._check(self.lib.clv_autotune(self.h, arr, len(jobs), trial_samples, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in abi.Tuning._fields_ if k != "_pad"}
+
+    def render_batch(self, jobs, want_sq=False):
+        """clv_render_batch.  jobs: list of (scene_id, opts) or (scene_id, opts,
+        camera) with camera an abi.CameraInit replacing the scene's camera.
+        Returns [(xyz[H, W, 3], sq or None)] in job order."""
+        arr, outs = self._jobs(jobs, want_sq)
         self._check(self.lib.clv_render_batch(self.h, arr, len(jobs)))
         return outs
 

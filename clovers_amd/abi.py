@@ -78,6 +78,15 @@ class Job(C.Structure):
 assert C.sizeof(Job) == 128, C.sizeof(Job)
 
 
+TRACE_SIMPLE, TRACE_PERSISTENT, TRACE_WW, TRACE_LDS_CACHED = range(4)
+TRACE_NAMES = {0: "simple", 1: "persistent", 2: "ww", 3: "lds_cached"}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("target_paths", C.c_uint32), ("trace_kernel", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("shade_grid_cap", C.c_uint32), ("_pad", C.c_uint32 * 4)]
+
+
 class Counters(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("prim_tests", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),

@@ -32,6 +32,10 @@
 
 using namespace clv;
 
+static_assert(CLV_TRACE_SIMPLE == TRACE_SIMPLE && CLV_TRACE_PERSISTENT == TRACE_PERSISTENT && CLV_TRACE_WW == TRACE_WW &&
+                  CLV_TRACE_LDS_CACHED == TRACE_LDS,
+              "public and internal traversal kernel ids differ");
+
 namespace {
 
 /* one growable device array of the scene arena */
@@ -722,6 +726,96 @@ clv_status clv_render(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, f
 clv_status clv_render_batch(clv_ctx *c, const clv_job *jobs, uint32_t n_jobs) {
   if (!c) return CLV_EINVAL;
   return render_jobs(c, jobs, n_jobs);
+}
+
+clv_status clv_tuning_get(const clv_ctx *c, clv_tuning *out) {
+  if (!c || !out) return CLV_EINVAL;
+  memset(out, 0, sizeof(*out));
+  out->target_paths = c->target_paths;
+  out->trace_kernel = (uint32_t)c->trace_variant;
+  out->lds_bytes = (uint32_t)c->lds_block_bytes;
+  out->shade_grid_cap = c->grid_cap;
+  return CLV_OK;
+}
+
+clv_status clv_tuning_set(clv_ctx *c, const clv_tuning *t) {
+  if (!c) return CLV_EINVAL;
+  if (!t) return fail(c, CLV_EINVAL, "tuning is NULL");
+  if (t->target_paths < 65536 || t->target_paths > (1u << 28)) return fail(c, CLV_EINVAL, "target_paths outside [2^16, 2^28]");
+  if (t->trace_kernel > CLV_TRACE_LDS_CACHED) return fail(c, CLV_EINVAL, "unknown trace kernel");
+  if (t->lds_bytes > 64512) return fail(c, CLV_EINVAL, "lds_bytes above 64512");
+  if (t->shade_grid_cap < 1) return fail(c, CLV_EINVAL, "shade_grid_cap must be positive");
+  c->target_paths = t->target_paths;
+  c->trace_variant = (int)t->trace_kernel;
+  c->lds_block_bytes = t->lds_bytes;
+  c->grid_cap = t->shade_grid_cap;
+  return CLV_OK;
+}
+
+clv_status clv_autotune(clv_ctx *c, const clv_job *jobs, uint32_t n, uint64_t trial_samples, clv_tuning *best_out) {
+  if (!c) return CLV_EINVAL;
+  if (!jobs || n == 0 || n > MAX_JOBS) return fail(c, CLV_EINVAL, "no jobs");
+  if (trial_samples == 0) trial_samples = 1ull << 26;
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < n; k++) {
+    CHK(check_opts(c, &jobs[k].opts));
+    total += (uint64_t)jobs[k].opts.width * jobs[k].opts.height * jobs[k].opts.samples;
+  }
+  /* the shortened batch: every job keeps its shape, with its samples scaled */
+  std::vector<clv_job> trial(jobs, jobs + n);
+  std::vector<std::vector<float>> bufs(n);
+  uint64_t tsamples = 0;
+  for (uint32_t k = 0; k < n; k++) {
+    clv_render_opts &o = trial[k].opts;
+    uint64_t spp = total > trial_samples ? (uint64_t)o.samples * trial_samples / total : o.samples;
+    o.samples = (uint32_t)std::max<uint64_t>(1, spp);
+    o.flags = 0;
+    bufs[k].resize(3 * (size_t)o.width * o.height);
+    trial[k].xyz_out = bufs[k].data();
+    trial[k].xyz_sq_out = nullptr;
+    tsamples += (uint64_t)o.width * o.height * o.samples;
+  }
+  clv_tuning start;
+  CHK(clv_tuning_get(c, &start));
+  struct Cand {
+    uint32_t target, kernel, lds;
+  };
+  std::vector<Cand> cands;
+  for (uint32_t lg = 22; lg <= 26; lg++) {
+    const uint32_t tgt = 1u << lg;
+    if (lg > 22 && (uint64_t)(tgt >> 1) >= tsamples) break; /* larger passes than the work change nothing */
+    cands.push_back({tgt, CLV_TRACE_LDS_CACHED, 64512});
+    cands.push_back({tgt, CLV_TRACE_LDS_CACHED, 16384});
+    cands.push_back({tgt, CLV_TRACE_SIMPLE, 64512});
+  }
+  clv_tuning best = start;
+  double best_ms = 1e300;
+  bool warmed = false;
+  for (const Cand &cd : cands) {
+    clv_tuning t = start;
+    t.target_paths = cd.target;
+    t.trace_kernel = cd.kernel;
+    t.lds_bytes = cd.lds;
+    CHK(clv_tuning_set(c, &t));
+    clv_status st = render_jobs(c, trial.data(), n);
+    if (st == CLV_ENOMEM) continue; /* too large a pass for the device: skip */
+    if (st != CLV_OK) {
+      (void)clv_tuning_set(c, &start);
+      return st;
+    }
+    if (!warmed) { /* first trial also pays allocation and first-launch costs: repeat it */
+      warmed = true;
+      CHK(render_jobs(c, trial.data(), n));
+    }
+    const double ms = c->counters.render_ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  CHK(clv_tuning_set(c, &best));
+  if (best_out) *best_out = best;
+  return CLV_OK;
 }
 
 clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, uint32_t *ids, float *ts) {

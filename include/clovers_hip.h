@@ -243,6 +243,31 @@ typedef struct {
  * samples). */
 clv_status clv_render_batch(clv_ctx *ctx, const clv_job *jobs, uint32_t n_jobs);
 
+/* Launch shapes of a context.  No reference counterpart (the reference has no
+ * device launches); set by clv_autotune or by hand. */
+typedef enum {
+  CLV_TRACE_SIMPLE = 0,     /* one ray per thread, 256-thread blocks, stack top in LDS */
+  CLV_TRACE_PERSISTENT = 1, /* persistent waves refilling lanes from a work counter */
+  CLV_TRACE_WW = 2,         /* one ray per thread, node steps and leaf steps in separate loops */
+  CLV_TRACE_LDS_CACHED = 3  /* 768-thread blocks, top of the BVH and leading records cached in LDS */
+} clv_trace_kernel;
+typedef struct {
+  uint32_t target_paths;    /* paths in flight per wavefront pass (queue and path-state capacity) */
+  uint32_t trace_kernel;    /* clv_trace_kernel */
+  uint32_t lds_bytes;       /* dynamic LDS per block of CLV_TRACE_LDS_CACHED (<= 64512) */
+  uint32_t shade_grid_cap;  /* largest grid of the shading kernel, in 256-thread blocks */
+  uint32_t _pad[4];
+} clv_tuning;
+clv_status clv_tuning_get(const clv_ctx *ctx, clv_tuning *out);
+clv_status clv_tuning_set(clv_ctx *ctx, const clv_tuning *t);
+/* Autotuner: renders a shortened copy of the batch (the same jobs with fewer
+ * samples, about `trial_samples` in all; 0 = 64M) under each candidate launch
+ * shape -- pass sizes 2^22..2^26 paths x {LDS-cached traversal with a 64 KB or
+ * 16 KB cache, plain traversal} -- keeps the fastest on the context and returns
+ * it.  Outputs of the trials are discarded. */
+clv_status clv_autotune(clv_ctx *ctx, const clv_job *jobs, uint32_t n_jobs, uint64_t trial_samples,
+                        clv_tuning *best_out);
+
 /* Primary-ray closest hits for sample `opts->sample_offset` of every pixel,
  * same camera and ray seeds as clv_render (the parity probe of the hit path).
  * prim_id_out: W*H ids, object_index*8 + face (face = 0..5 for a Boxy side,

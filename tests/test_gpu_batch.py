@@ -97,3 +97,35 @@ def test_batch_errors():
             r.render_batch([(a, abi.render_opts(8, 8, 0, 4))])
         assert e.value.status == abi.EINVAL
         assert r.lib.clv_render_batch(r.h, None, 0) == abi.EINVAL
+
+
+def test_launch_shapes_do_not_change_results():
+    """Pass size and traversal kernel are launch shapes only: every tuning gives
+    the same bytes."""
+    with clovers_amd.Renderer(0) as r:
+        a = r.upload(synth.spheres())
+        b = r.upload(tiny_mesh())
+        jobs = [(a, abi.render_opts(120, 80, 6, 16, seed=4)), (b, abi.render_opts(96, 64, 5, 12, seed=5))]
+        want = [img for img, _ in r.render_batch(jobs)]
+        for t in [dict(target_paths=65536, trace_kernel=abi.TRACE_SIMPLE),
+                  dict(target_paths=1 << 20, trace_kernel=abi.TRACE_LDS_CACHED, lds_bytes=16384),
+                  dict(target_paths=1 << 22, trace_kernel=abi.TRACE_WW),
+                  dict(target_paths=1 << 18, trace_kernel=abi.TRACE_PERSISTENT, shade_grid_cap=64)]:
+            r.set_tuning(**t)
+            got = [img for img, _ in r.render_batch(jobs)]
+            for g, w_ in zip(got, want):
+                assert np.array_equal(g, w_), t
+
+
+def test_autotune_keeps_a_valid_tuning():
+    with clovers_amd.Renderer(0) as r:
+        a = r.upload(synth.cornell())
+        jobs = [(a, abi.render_opts(256, 256, 16, 8, seed=1))]
+        best = r.autotune(jobs, trial_samples=1 << 20)
+        assert best == r.tuning()
+        assert best["trace_kernel"] in (abi.TRACE_SIMPLE, abi.TRACE_LDS_CACHED)
+        assert best["target_paths"] >= 1 << 22
+        img, _ = r.render_batch(jobs)[0]
+        assert np.isfinite(img).all()
+        with pytest.raises(clovers_amd.ClvError):
+            r.set_tuning(target_paths=5)
