@@ -59,6 +59,16 @@ def load_library(path=LIB_PATH):
     lib.clv_tuning_set.argtypes = [vp, vp]
     lib.clv_autotune.restype = C.c_int
     lib.clv_autotune.argtypes = [vp, vp, C.c_uint32, C.c_uint64, vp]
+    lib.clv_guide_reset.restype = C.c_int
+    lib.clv_guide_reset.argtypes = [vp, C.c_uint64]
+    lib.clv_guide_params.restype = C.c_int
+    lib.clv_guide_params.argtypes = [vp, vp, C.c_uint32]
+    lib.clv_guide_set_params.restype = C.c_int
+    lib.clv_guide_set_params.argtypes = [vp, vp, C.c_uint32]
+    lib.clv_guide_eval.restype = C.c_int
+    lib.clv_guide_eval.argtypes = [vp, vp, C.c_uint32, vp, vp]
+    lib.clv_guide_grad.restype = C.c_int
+    lib.clv_guide_grad.argtypes = [vp, vp, C.c_uint32, vp]
     lib.clv_primary_hits.restype = C.c_int
     lib.clv_primary_hits.argtypes = [vp, C.c_uint32, vp, vp, vp]
     lib.clv_counters_get.restype = C.c_int
@@ -188,6 +198,35 @@ class Renderer:
         t = abi.Tuning()
         self._check(self.lib.clv_autotune(self.h, arr, len(jobs), trial_samples, C.byref(t)))
         return {k: getattr(t, k) for k, _ in abi.Tuning._fields_ if k != "_pad"}
+
+    # -- guiding network probes
+    def guide_reset(self, seed=1):
+        self._check(self.lib.clv_guide_reset(self.h, seed))
+
+    def guide_params(self):
+        p = np.zeros(abi.GUIDE_NPARAM, np.float32)
+        self._check(self.lib.clv_guide_params(self.h, p.ctypes.data, p.size))
+        return p
+
+    def guide_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self.lib.clv_guide_set_params(self.h, p.ctypes.data, p.size))
+
+    def guide_eval(self, inputs):
+        """inputs [n, 7] -> (raw outputs [n, 40], lobes [n, 8, 5])"""
+        x = np.ascontiguousarray(inputs, np.float32)
+        n = x.shape[0]
+        raw = np.zeros((n, abi.GUIDE_OUT), np.float32)
+        lob = np.zeros((n, abi.GUIDE_OUT), np.float32)
+        self._check(self.lib.clv_guide_eval(self.h, x.ctypes.data, n, raw.ctypes.data, lob.ctypes.data))
+        return raw, lob.reshape(n, 8, 5)
+
+    def guide_grad(self, samples):
+        """samples [n, 11] -> mean loss gradient [GUIDE_NPARAM]"""
+        x = np.ascontiguousarray(samples, np.float32)
+        g = np.zeros(abi.GUIDE_NPARAM, np.float32)
+        self._check(self.lib.clv_guide_grad(self.h, x.ctypes.data, x.shape[0], g.ctypes.data))
+        return g
 
     def render_batch(self, jobs, want_sq=False):
         """clv_render_batch.  jobs: list of (scene_id, opts) or (scene_id, opts,

@@ -84,7 +84,9 @@ TRACE_NAMES = {0: "simple", 1: "persistent", 2: "ww", 3: "lds_cached"}
 
 class Tuning(C.Structure):
     _fields_ = [("target_paths", C.c_uint32), ("trace_kernel", C.c_uint32), ("lds_bytes", C.c_uint32),
-                ("shade_grid_cap", C.c_uint32), ("_pad", C.c_uint32 * 4)]
+                ("shade_grid_cap", C.c_uint32), ("guide_train_spp", C.c_uint32), ("guide_alpha", C.c_float),
+                ("guide_lr", C.c_float), ("guide_records", C.c_uint32), ("guide_wclip", C.c_float),
+                ("guide_rounds", C.c_uint32), ("_pad", C.c_uint32 * 2)]
 
 
 class Counters(C.Structure):
@@ -103,6 +105,10 @@ class SceneInfo(C.Structure):
 
 FLAG_TIME_KERNELS = 1
 FLAG_COUNT = 2
+FLAG_GUIDE_FROZEN = 4
+FLAG_GUIDE_KEEP = 8
+GUIDE_NPARAM = 15600
+GUIDE_OUT = 40
 
 
 def camera_init(look_from, look_at, up=(0.0, 1.0, 0.0), vertical_fov=40.0, aperture=0.0, focus_distance=10.0,

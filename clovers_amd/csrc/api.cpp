@@ -16,6 +16,8 @@
  * sequence of a pass is fixed and the driver pipelines it.
  */
 #include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -56,8 +58,36 @@ struct SceneRes {
   clv_camera_init camera;
   uint32_t n_nodes = 0, n_prims = 0, prim_bytes = 0, depth = 0;
   uint32_t prim_units = 0;   /* float4 units of primitive records */
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  /* scene box (root node's child boxes) */
   double build_ms = 0.0;
 };
+
+/* the guiding network and its training buffers (allocated on first use) */
+struct GuideState {
+  float *P = nullptr, *M = nullptr, *V = nullptr;  /* fp32 parameters, Adam moments */
+  _Float16 *Wh = nullptr;                          /* fp16 operand copies */
+  float *partial = nullptr;                        /* per-wave gradient slices */
+  uint32_t partial_waves = 0;
+  uint32_t cap_paths = 0;                          /* path capacity of the buffers below */
+  float *lobes = nullptr;
+  GuideRec *rec = nullptr;
+  uint32_t *gcnt = nullptr;
+  uint32_t *list = nullptr;
+  uint32_t *bcount = nullptr;
+  float *bweight = nullptr;
+  uint32_t *totals = nullptr;
+  uint32_t t = 0;                                  /* Adam steps taken */
+  uint32_t stamp = 0;                              /* last pass id */
+  bool init = false, trained = false;
+  /* training schedule (clv_tuning) */
+  uint32_t train_spp = 4;
+  float alpha = 0.9f;
+  float lr = 0.005f;
+  uint32_t records = 262144;
+  float wclip = 1e6f;
+  uint32_t rounds = 16;
+};
+#define GUIDE_STEP 16384u
 
 }  // namespace
 
@@ -99,6 +129,7 @@ struct clv_ctx {
   clv_counters counters;
   uint32_t target_paths = 1u << 22;
   uint32_t grid_cap = 2048;
+  GuideState guide;
 };
 
 namespace {
@@ -306,6 +337,10 @@ DevJob make_job(const SceneRes *s, const clv_camera_init &cam, const clv_render_
   j.spp = o->samples;
   j.fp64 = o->fp64_accum ? 1u : 0u;
   j.guiding = o->guiding;
+  for (int a = 0; a < 3; a++) {
+    j.lo[a] = s->lo[a];
+    j.hi[a] = s->hi[a];
+  }
   return j;
 }
 
@@ -319,16 +354,18 @@ struct PassPlan {
   std::vector<uint32_t> paths;      /* per pass: paths */
   std::vector<uint32_t> res;        /* per pass: resolve threads */
   std::vector<uint32_t> depth;      /* per pass: largest max_depth of its jobs */
+  std::vector<uint8_t> train;       /* per pass: holds a training slice of a guided job */
   uint32_t max_paths = 0;
 };
 
 void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &sample_offset, uint32_t target,
-                 PassPlan *pl) {
+                 uint32_t guided_cap, uint32_t rounds, PassPlan *pl) {
   const uint32_t n = (uint32_t)jobs.size();
-  std::vector<uint32_t> done(n, 0);
+  std::vector<uint32_t> done(n, 0), trained(n, 0);
   uint32_t j = 0;
   while (j < n) {
     uint32_t budget = target, paths = 0, res = 0, depth = 0;
+    bool train = false;
     pl->first.push_back((uint32_t)pl->items.size());
     while (j < n) {
       const uint32_t pix = jobs[j].width * jobs[j].height;
@@ -339,6 +376,8 @@ void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &s
         kk = 1;
       }
       if (kk > left) kk = left;
+      const bool training = jobs[j].guiding && trained[j] < rounds;
+      if (training && kk > guided_cap) kk = guided_cap; /* a training round after every such slice */
       DevItem it;
       memset(&it, 0, sizeof(it));
       it.job = j;
@@ -353,15 +392,113 @@ void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &s
       depth = std::max(depth, jobs[j].max_depth);
       budget = budget > kk * pix ? budget - kk * pix : 0;
       done[j] += kk;
+      if (training) {
+        trained[j]++;
+        train = true;
+      }
       if (done[j] == jobs[j].spp) j++;
-      if (budget == 0) break;
+      if (budget == 0 || training) break; /* a training slice ends its pass */
     }
     pl->paths.push_back(paths);
     pl->res.push_back(res);
     pl->depth.push_back(depth);
+    pl->train.push_back(train ? 1 : 0);
     pl->max_paths = std::max(pl->max_paths, paths);
   }
   pl->first.push_back((uint32_t)pl->items.size());
+}
+
+/* ------------------------------------------------------------ guiding --- */
+clv_status guide_ensure_net(clv_ctx *c) {
+  GuideState &g = c->guide;
+  if (g.P) return CLV_OK;
+  HIPCHK(c, dalloc(&g.P, GD_NPARAM));
+  HIPCHK(c, dalloc(&g.M, GD_NPARAM));
+  HIPCHK(c, dalloc(&g.V, GD_NPARAM));
+  HIPCHK(c, dalloc(&g.Wh, GD_NHALF));
+  HIPCHK(c, dalloc(&g.totals, 4));
+  return CLV_OK;
+}
+
+clv_status guide_ensure_partial(clv_ctx *c, uint32_t waves) {
+  GuideState &g = c->guide;
+  if (waves <= g.partial_waves) return CLV_OK;
+  dfree(g.partial);
+  g.partial = nullptr;
+  g.partial_waves = 0;
+  HIPCHK(c, dalloc(&g.partial, (size_t)(waves + (waves + 15u) / 16u) * GD_NPARAM)); /* wave slices + chunk sums */
+  g.partial_waves = waves;
+  return CLV_OK;
+}
+
+void guide_free(GuideState &g) {
+  dfree(g.P); dfree(g.M); dfree(g.V); dfree(g.Wh); dfree(g.partial); dfree(g.totals);
+  dfree(g.lobes); dfree(g.rec); dfree(g.gcnt); dfree(g.list); dfree(g.bcount); dfree(g.bweight);
+}
+
+clv_status guide_ensure_paths(clv_ctx *c, uint32_t paths) {
+  GuideState &g = c->guide;
+  if (paths <= g.cap_paths) return CLV_OK;
+  dfree(g.lobes); dfree(g.rec); dfree(g.gcnt); dfree(g.list); dfree(g.bcount); dfree(g.bweight);
+  g.lobes = nullptr; g.rec = nullptr; g.gcnt = nullptr; g.list = nullptr; g.bcount = nullptr; g.bweight = nullptr;
+  g.cap_paths = 0;
+  const size_t slots = (size_t)paths * GD_SLOTS;
+  HIPCHK(c, dalloc(&g.lobes, (size_t)paths * GD_LFLOATS));
+  HIPCHK(c, dalloc(&g.rec, slots));
+  HIPCHK(c, dalloc(&g.gcnt, paths));
+  HIPCHK(c, dalloc(&g.list, slots));
+  HIPCHK(c, dalloc(&g.bcount, slots / 256 + 1));
+  HIPCHK(c, dalloc(&g.bweight, slots / 256 + 1));
+  /* stale stamps must never match a future pass id */
+  HIPCHK(c, hipMemsetAsync(g.rec, 0, slots * sizeof(GuideRec), c->stream));
+  g.cap_paths = paths;
+  return CLV_OK;
+}
+
+/* fresh parameters: He-normal hidden layers, a small output layer (broad,
+ * nearly uniform lobes), zero biases, zero Adam moments */
+clv_status guide_reset(clv_ctx *c, uint64_t seed) {
+  CHK(guide_ensure_net(c));
+  GuideState &g = c->guide;
+  std::vector<float> P(GD_NPARAM, 0.0f);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto next = [&st]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  auto normal = [&]() { return sqrt(-2.0 * log(next())) * cos(6.283185307179586 * next()); };
+  const float he = sqrtf(2.0f / 64.0f);
+  for (int k = 0; k < GD_HID * GD_IN; k++) P[GD_W1 + k] = (float)normal() * he;
+  for (int k = 0; k < GD_HID * GD_HID; k++) P[GD_W2 + k] = (float)normal() * he;
+  for (int k = 0; k < GD_HID * GD_HID; k++) P[GD_W3 + k] = (float)normal() * he;
+  for (int k = 0; k < GD_OUT * GD_HID; k++) P[GD_W4 + k] = (float)normal() * 0.01f;
+  HIPCHK(c, hipMemcpyAsync(g.P, P.data(), GD_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(g.M, 0, GD_NPARAM * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(g.V, 0, GD_NPARAM * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(g.Wh, 0, GD_NHALF * sizeof(_Float16), c->stream));
+  launch_guide_halves(c->stream, g.P, g.Wh);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  g.t = 0;
+  g.init = true;
+  g.trained = false;
+  return CLV_OK;
+}
+
+/* one training round on the records of pass `stamp` (n path slots) */
+void guide_train_round(clv_ctx *c, uint32_t paths, uint32_t stamp, float *grad_out, float wclip) {
+  GuideState &g = c->guide;
+  launch_guide_collect(c->stream, g.rec, paths * GD_SLOTS, stamp, g.bcount, g.bweight, g.totals, g.list);
+  const uint32_t cap = grad_out ? paths * GD_SLOTS : g.records;
+  const uint32_t step = grad_out ? cap : GUIDE_STEP;
+  for (uint32_t b = 0; b < cap; b += step) {
+    if (!grad_out) g.t++;
+    launch_guide_step(c->stream, g.rec, g.list, g.totals, cap, b, std::min(step, cap - b), wclip, g.P, g.M, g.V,
+                      g.Wh, g.partial, g.lr, std::max<uint32_t>(g.t, 1), grad_out);
+  }
 }
 
 /* the batch render behind clv_render and clv_render_batch */
@@ -379,7 +516,6 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     CHK(check_opts(c, &jb.opts));
     if (!jb.xyz_out) return fail(c, CLV_EINVAL, "xyz_out is NULL");
     if (jb.opts.samples == 0) return fail(c, CLV_EINVAL, "samples must be positive");
-    if (jb.opts.guiding == CLV_GUIDE_ON) return fail(c, CLV_EUNSUPPORTED, "guiding is not available in this build");
     auto it = c->scenes.find(jb.scene_id);
     if (it == c->scenes.end()) return fail(c, CLV_EINVAL, "unknown scene id");
     sres[k] = it->second;
@@ -392,13 +528,27 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     any_xform = any_xform || it->second->has_xforms;
   }
   if (pixels > (1ull << 28)) return fail(c, CLV_EINVAL, "batch larger than 2^28 pixels");
+  bool any_guide = false;
+  uint64_t guide_seed = 0;
+  for (uint32_t k = 0; k < n; k++)
+    if (jobs[k].opts.guiding == CLV_GUIDE_ON && !any_guide) {
+      any_guide = true;
+      guide_seed = jobs[k].opts.seed;
+    }
   HIPCHK(c, hipSetDevice(c->device));
   PassPlan pl;
-  plan_passes(dj, soff, c->target_paths, &pl);
+  plan_passes(dj, soff, c->target_paths, std::max<uint32_t>(1, c->guide.train_spp), c->guide.rounds, &pl);
   const uint32_t passes = (uint32_t)pl.paths.size();
   CHK(ensure_paths(c, pl.max_paths));
   CHK(ensure_pixels(c, (uint32_t)pixels));
   CHK(ensure_tables(c, n, (uint32_t)pl.items.size()));
+  GuideState &gs = c->guide;
+  const bool frozen = (flags & CLV_FLAG_GUIDE_FROZEN) != 0;
+  if (any_guide) {
+    CHK(guide_ensure_paths(c, pl.max_paths));
+    CHK(guide_ensure_partial(c, GUIDE_STEP / 64u));
+    if (!(flags & CLV_FLAG_GUIDE_KEEP) || !gs.init) CHK(guide_reset(c, guide_seed));
+  }
   /* the LDS window caches the scene with the most samples */
   uint32_t big = 0;
   for (uint32_t k = 1; k < n; k++)
@@ -435,6 +585,17 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   launch_zero_accum(c->stream, rp);
   HIPCHK(c, hipEventRecord(c->events[0], c->stream));
   size_t ev = 2;
+  PathBuf pb = c->pb;
+  pb.gcnt = any_guide ? gs.gcnt : nullptr;
+  GuideArgs ga;
+  memset(&ga, 0, sizeof(ga));
+  if (any_guide) {
+    ga.lobes = gs.lobes;
+    ga.rec = gs.rec;
+    ga.Wh = gs.Wh;
+    ga.P = gs.P;
+    ga.on = 1u;
+  }
   for (uint32_t p = 0; p < passes; p++) {
     GenParams gp;
     memset(&gp, 0, sizeof(gp));
@@ -442,7 +603,13 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     gp.n_items = pl.first[p + 1] - pl.first[p];
     gp.paths = pl.paths[p];
     launch_set_count(c->stream, c->counts, CNT_QA, gp.paths);
-    launch_generate(c->stream, ds, c->pb, gp, c->queue[0]);
+    launch_generate(c->stream, ds, pb, gp, c->queue[0]);
+    if (any_guide) {
+      /* plain sampling until the network has had a training round (or, frozen, always the network) */
+      ga.alpha = (gs.trained || frozen) ? gs.alpha : 0.0f;
+      ga.record = (!frozen && pl.train[p]) ? 1u : 0u;
+      ga.stamp = ++gs.stamp;
+    }
     for (uint32_t depth = 0; depth <= pl.depth[p]; depth++) {
       uint32_t in = depth & 1u, out = in ^ 1u;
       uint32_t in_slot = in ? CNT_QB : CNT_QA, out_slot = out ? CNT_QB : CNT_QA;
@@ -458,17 +625,23 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
       ta.stats = (unsigned long long *)c->stats;
       fill_trace_args(&ta, geom);
       if (c->trace_variant == TRACE_LDS)
-        launch_trace_lds(c->stream, ds, c->pb, ta, geom.blocks, geom.lds_bytes, counting, 0);
+        launch_trace_lds(c->stream, ds, pb, ta, geom.blocks, geom.lds_bytes, counting, 0);
       else
-        launch_extend(c->stream, ds, c->pb, ta, geom.blocks, counting, c->trace_variant);
+        launch_extend(c->stream, ds, pb, ta, geom.blocks, counting, c->trace_variant);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
       ev += timing ? 2 : 0;
-      launch_shade(c->stream, ds, c->pb, sq, c->queue[out], c->counts, out_slot, gp.paths, c->grid_cap);
+      if (any_guide && ga.alpha > 0.0f)
+        launch_guide_infer(c->stream, ds, pb, ga, sq.q[MC_LAMBERTIAN], sq.counts + MC_LAMBERTIAN, gp.paths);
+      launch_shade(c->stream, ds, pb, sq, c->queue[out], c->counts, out_slot, gp.paths, c->grid_cap, ga);
     }
     rp.items = gp.items;
     rp.n_items = gp.n_items;
     rp.threads = pl.res[p];
-    launch_resolve(c->stream, c->pb, rp, c->cie);
+    launch_resolve(c->stream, pb, rp, c->cie);
+    if (any_guide && ga.record) {
+      guide_train_round(c, gp.paths, ga.stamp, nullptr, gs.wclip);
+      gs.trained = true;
+    }
   }
   bool any_sq = false;
   for (uint32_t k = 0; k < n; k++) any_sq = any_sq || jobs[k].xyz_sq_out;
@@ -587,6 +760,7 @@ void clv_destroy(clv_ctx *c) {
   free_pixels(c);
   dfree(c->jobs);
   dfree(c->items);
+  guide_free(c->guide);
   dfree(c->counts);
   dfree(c->stats);
   dfree(c->spill);
@@ -660,6 +834,21 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
   s->prim_bytes = hs.n_prims ? (uint32_t)(hs.prims.size() * 4 / hs.n_prims) : 0;
   s->depth = hs.bvh_depth;
   s->build_ms = hs.build_ms;
+  {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    const Bvh4Node &root = hs.nodes[0];
+    for (int k = 0; k < 4; k++) {
+      if (root.child[k] == CHILD_EMPTY) continue;
+      lo[0] = std::min(lo[0], root.lo_x[k]); hi[0] = std::max(hi[0], root.hi_x[k]);
+      lo[1] = std::min(lo[1], root.lo_y[k]); hi[1] = std::max(hi[1], root.hi_y[k]);
+      lo[2] = std::min(lo[2], root.lo_z[k]); hi[2] = std::max(hi[2], root.hi_z[k]);
+    }
+    for (int a = 0; a < 3; a++) {
+      if (!(hi[a] - lo[a] > 1e-6f)) { lo[a] -= 1.0f; hi[a] += 1.0f; }
+      s->lo[a] = lo[a];
+      s->hi[a] = hi[a];
+    }
+  }
   c->counters.build_ms = hs.build_ms;
   uint32_t id = c->next_scene++;
   c->scenes[id] = s;
@@ -735,6 +924,12 @@ clv_status clv_tuning_get(const clv_ctx *c, clv_tuning *out) {
   out->trace_kernel = (uint32_t)c->trace_variant;
   out->lds_bytes = (uint32_t)c->lds_block_bytes;
   out->shade_grid_cap = c->grid_cap;
+  out->guide_train_spp = c->guide.train_spp;
+  out->guide_alpha = c->guide.alpha;
+  out->guide_lr = c->guide.lr;
+  out->guide_records = c->guide.records;
+  out->guide_wclip = c->guide.wclip;
+  out->guide_rounds = c->guide.rounds;
   return CLV_OK;
 }
 
@@ -745,10 +940,122 @@ clv_status clv_tuning_set(clv_ctx *c, const clv_tuning *t) {
   if (t->trace_kernel > CLV_TRACE_LDS_CACHED) return fail(c, CLV_EINVAL, "unknown trace kernel");
   if (t->lds_bytes > 64512) return fail(c, CLV_EINVAL, "lds_bytes above 64512");
   if (t->shade_grid_cap < 1) return fail(c, CLV_EINVAL, "shade_grid_cap must be positive");
+  if (t->guide_train_spp < 1) return fail(c, CLV_EINVAL, "guide_train_spp must be positive");
+  if (!(t->guide_alpha > 0.0f && t->guide_alpha < 1.0f)) return fail(c, CLV_EINVAL, "guide_alpha outside (0, 1)");
+  if (!(t->guide_lr > 0.0f && t->guide_lr < 1.0f)) return fail(c, CLV_EINVAL, "guide_lr outside (0, 1)");
+  if (t->guide_records < 64 || t->guide_records > (1u << 24)) return fail(c, CLV_EINVAL, "guide_records outside [64, 2^24]");
+  if (!(t->guide_wclip >= 1.0f)) return fail(c, CLV_EINVAL, "guide_wclip below 1");
   c->target_paths = t->target_paths;
   c->trace_variant = (int)t->trace_kernel;
   c->lds_block_bytes = t->lds_bytes;
   c->grid_cap = t->shade_grid_cap;
+  c->guide.train_spp = t->guide_train_spp;
+  c->guide.alpha = t->guide_alpha;
+  c->guide.lr = t->guide_lr;
+  c->guide.records = t->guide_records;
+  c->guide.wclip = t->guide_wclip;
+  c->guide.rounds = t->guide_rounds;
+  return CLV_OK;
+}
+
+clv_status clv_guide_reset(clv_ctx *c, uint64_t seed) {
+  if (!c) return CLV_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  return guide_reset(c, seed);
+}
+
+clv_status clv_guide_params(clv_ctx *c, float *out, uint32_t n) {
+  if (!c) return CLV_EINVAL;
+  if (!out || n < GD_NPARAM) return fail(c, CLV_EINVAL, "params_out needs CLV_GUIDE_NPARAM floats");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->guide.init) CHK(guide_reset(c, 1));
+  HIPCHK(c, hipMemcpyAsync(out, c->guide.P, GD_NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLV_OK;
+}
+
+clv_status clv_guide_set_params(clv_ctx *c, const float *in, uint32_t n) {
+  if (!c) return CLV_EINVAL;
+  if (!in || n != GD_NPARAM) return fail(c, CLV_EINVAL, "params need exactly CLV_GUIDE_NPARAM floats");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->guide.init) CHK(guide_reset(c, 1));
+  GuideState &g = c->guide;
+  HIPCHK(c, hipMemcpyAsync(g.P, in, GD_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(g.M, 0, GD_NPARAM * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(g.V, 0, GD_NPARAM * sizeof(float), c->stream));
+  launch_guide_halves(c->stream, g.P, g.Wh);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  g.t = 0;
+  g.trained = true;
+  return CLV_OK;
+}
+
+clv_status clv_guide_eval(clv_ctx *c, const float *in, uint32_t n, float *raw, float *lobes) {
+  if (!c) return CLV_EINVAL;
+  if (!in || n == 0 || n > (1u << 24)) return fail(c, CLV_EINVAL, "inputs: 1..2^24 rows of 7 floats");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->guide.init) CHK(guide_reset(c, 1));
+  float *din = nullptr, *draw = nullptr, *dlob = nullptr;
+  clv_status st = CLV_OK;
+  hipError_t e = dalloc(&din, 7 * (size_t)n);
+  if (e == hipSuccess) e = dalloc(&draw, (size_t)GD_OUT * n);
+  if (e == hipSuccess) e = dalloc(&dlob, (size_t)GD_OUT * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(din, in, 7 * sizeof(float) * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_guide_eval(c->stream, din, n, c->guide.Wh, c->guide.P, draw, dlob);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && raw) e = hipMemcpyAsync(raw, draw, sizeof(float) * GD_OUT * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && lobes) e = hipMemcpyAsync(lobes, dlob, sizeof(float) * GD_OUT * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) st = fail(c, e == hipErrorOutOfMemory ? CLV_ENOMEM : CLV_EDEVICE, hipGetErrorString(e));
+  dfree(din);
+  dfree(draw);
+  dfree(dlob);
+  return st;
+}
+
+clv_status clv_guide_grad(clv_ctx *c, const float *samples, uint32_t n, float *grad_out) {
+  if (!c) return CLV_EINVAL;
+  if (!samples || !grad_out || n == 0 || n > (1u << 20)) return fail(c, CLV_EINVAL, "samples: 1..2^20 rows of 11 floats");
+  for (uint32_t i = 0; i < n; i++)
+    if (!(samples[11 * (size_t)i + 10] > 0.0f)) return fail(c, CLV_EINVAL, "sample weights must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->guide.init) CHK(guide_reset(c, 1));
+  GuideState &g = c->guide;
+  CHK(guide_ensure_paths(c, (n + GD_SLOTS - 1) / GD_SLOTS));
+  CHK(guide_ensure_partial(c, (n + 63) / 64));
+  const uint32_t stamp = ++g.stamp;
+  std::vector<GuideRec> recs(n);
+  for (uint32_t i = 0; i < n; i++) {
+    const float *sm = samples + 11 * (size_t)i;
+    GuideRec &r = recs[i];
+    memset(&r, 0, sizeof(r));
+    for (int k = 0; k < 3; k++) {
+      r.p[k] = sm[k];
+      r.n[k] = sm[3 + k];
+      r.d[k] = sm[7 + k];
+    }
+    r.lam = sm[6];
+    r.pdf = 1.0f;
+    r.cosv = 1.0f;
+    r.t0 = 1.0f;
+    r.L = sm[10];
+    r.stamp = stamp;
+  }
+  float *dgrad = nullptr;
+  HIPCHK(c, dalloc(&dgrad, GD_NPARAM));
+  hipError_t e = hipMemcpyAsync(g.rec, recs.data(), n * sizeof(GuideRec), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_guide_collect(c->stream, g.rec, n, stamp, g.bcount, g.bweight, g.totals, g.list);
+    launch_guide_step(c->stream, g.rec, g.list, g.totals, n, 0, n, FLT_MAX, g.P, g.M, g.V, g.Wh, g.partial, g.lr, 1,
+                      dgrad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(grad_out, dgrad, GD_NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree(dgrad);
+  if (e != hipSuccess) return fail(c, CLV_EDEVICE, hipGetErrorString(e));
   return CLV_OK;
 }
 

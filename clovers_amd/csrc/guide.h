@@ -1,0 +1,299 @@
+/*
+ * guide.h -- the learned proposal of the guided sampler (device side).
+ *
+ * At a diffuse (Lambertian) vertex the plain method samples the reference's
+ * ½ light-tree + ½ cosine mixture (clovers-cli/src/trace.rs:80-120,
+ * clovers/src/pdf.rs:106-141).  The guided method keeps the light-tree half
+ * and splits the cosine half: with probability alpha the direction is drawn
+ * from a learned density q, otherwise from the cosine lobe, and the sample is
+ * weighted by the one-sample MIS mixture pdf
+ *     p(w) = ½ p_light(w) + ½ (alpha q(w) + (1 - alpha) p_cos(w)),
+ * so the estimator stays unbiased whatever q is, and direct light from small
+ * emitters keeps its exact area sampling.
+ *
+ * q is a mixture of GD_LOBES von Mises-Fisher lobes on the sphere,
+ *     q(w) = sum_j pi_j * kappa_j / (2 pi (1 - exp(-2 kappa_j))) * exp(kappa_j (mu_j . w - 1)),
+ * whose parameters come from a small MLP (3 hidden layers of 64, ReLU) of the
+ * vertex position (normalised to the scene box, frequency-encoded), the
+ * oriented normal and the hero wavelength.  The network runs on MFMA in fp16
+ * with fp32 accumulation; it is trained online on the tracer's own radiance
+ * samples by minimising the KL divergence from the target density
+ * proportional to L_i(w) * cos(theta) (Mueller et al. 2019, "Neural Importance
+ * Sampling"; vMF-mixture outputs as in Dong et al. 2023, "Neural Parametric
+ * Mixtures for Path Guiding"):
+ *     loss = - sum_s  (F_s / p_s) * log q(w_s),   F_s = L_s cos(theta_s).
+ *
+ * Matrix layout: every layer is computed transposed, out^T = W . in^T, so
+ * that the MFMA result of one layer (lane l holds rows 4*(l/16)+i of column
+ * l%16) is exactly the B operand of the next (lane l supplies k = 4*(l/16)+i
+ * of column l%16): columns are queries, activations never leave registers in
+ * the forward pass.
+ */
+#ifndef CLV_GUIDE_H
+#define CLV_GUIDE_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clv {
+
+#define GD_IN 64        /* encoded input features */
+#define GD_HID 64       /* hidden width */
+#define GD_LOBES 8
+#define GD_OUT 40       /* per lobe: mu (3, unnormalised), log kappa, mixture logit */
+#define GD_OUT_PAD 48   /* output rows padded to whole 16-row tiles */
+#define GD_FREQS 9      /* sin/cos of 2^k pi x, k < 9, per position axis */
+
+/* fp32 master parameter vector: W1 b1 W2 b2 W3 b3 W4 b4, W row-major [out][in] */
+#define GD_W1 0
+#define GD_B1 (GD_W1 + GD_HID * GD_IN)
+#define GD_W2 (GD_B1 + GD_HID)
+#define GD_B2 (GD_W2 + GD_HID * GD_HID)
+#define GD_W3 (GD_B2 + GD_HID)
+#define GD_B3 (GD_W3 + GD_HID * GD_HID)
+#define GD_W4 (GD_B3 + GD_HID)
+#define GD_B4 (GD_W4 + GD_OUT_PAD * GD_HID)
+#define GD_NPARAM (GD_B4 + GD_OUT_PAD)
+/* fp16 copies for the MFMA operands: W1..W4 row-major, then the transposes
+ * WT1..WT4 ([in][out], used by the backward pass) */
+#define GD_H_W(l) ((l) * GD_HID * GD_IN)                  /* l = 0..3; W4 has 48 rows */
+#define GD_H_WT(l) (GD_H_W(3) + GD_OUT_PAD * GD_HID + (l) * GD_HID * GD_IN)
+#define GD_NHALF (2 * (3 * GD_HID * GD_IN + GD_OUT_PAD * GD_HID))
+
+#define GD_LOGK_MIN -3.0f /* kappa in [0.05, 4915] */
+#define GD_LOGK_MAX 8.5f
+
+/* a training sample: one diffuse vertex of a traced path */
+struct GuideRec {
+  float p[3];       /* position, normalised to the scene box ([-1, 1]^3) */
+  float n[3];       /* oriented normal */
+  float d[3];       /* sampled direction */
+  float lam;        /* hero wavelength, (lambda - 380) / 400 */
+  float pdf;        /* density the direction was drawn with */
+  float cosv;       /* cos(theta) of the direction */
+  float t0;         /* hero throughput right after this vertex */
+  float L;          /* hero radiance that arrived along d afterwards (sum of c / t0) */
+  uint32_t stamp;   /* pass that wrote it */
+  uint32_t pad;
+};                  /* 64 bytes */
+
+#define GD_SLOTS 3  /* training records kept per path: its last 3 diffuse vertices */
+
+typedef _Float16 gh4 __attribute__((ext_vector_type(4)));
+typedef float gf4 __attribute__((ext_vector_type(4)));
+
+#define GD_PI 3.14159265358979323846f
+
+/* input feature `idx` (0..63) of a query.  sin(pi 2^k x) = sin(2 pi * 2^(k-1) x):
+ * the hardware sine takes revolutions and reduces them exactly (|2^(k-1) x| <= 128) */
+__device__ __forceinline__ float gd_feature(int idx, const float p[3], const float n[3], float lam) {
+  if (idx < 6 * GD_FREQS) {
+    const int k = idx / 6, r = idx - 6 * (idx / 6), ax = r % 3;
+    const float x = ax == 0 ? p[0] : (ax == 1 ? p[1] : p[2]);
+    const float rev = x * (0.5f * (float)(1 << k));
+    return r < 3 ? __builtin_amdgcn_sinf(rev) : __builtin_amdgcn_cosf(rev);
+  }
+  switch (idx) {
+    case 54: return p[0];
+    case 55: return p[1];
+    case 56: return p[2];
+    case 57: return n[0];
+    case 58: return n[1];
+    case 59: return n[2];
+    case 60: return lam;
+    case 61: return __builtin_amdgcn_sinf(lam);
+    case 62: return __builtin_amdgcn_cosf(lam);
+    default: return 1.0f;
+  }
+}
+
+/* B operands of the 4 k-steps of the first layer for the query whose inputs
+ * this lane sees: lane l supplies features 16*kt + 4*(l/16) + i */
+__device__ __forceinline__ void gd_encode(const float p[3], const float n[3], float lam, gh4 x[4]) {
+  const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int kt = 0; kt < 4; kt++) {
+    gh4 v;
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = (_Float16)gd_feature(16 * kt + 4 * g + i, p, n, lam);
+    x[kt] = v;
+  }
+}
+
+/* out^T[16*MT x 16] = W[16*MT x 64] . in^T[64 x 16] + b: W fp16 row-major with
+ * 64 columns, b fp32.  acc[mt] lane l: rows 16 mt + 4 (l/16) + i, column l%16. */
+template <int MT>
+__device__ __forceinline__ void gd_gemm(const _Float16 *__restrict__ W, const float *__restrict__ b, const gh4 in[4],
+                                        gf4 acc[MT]) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) {
+    gf4 c;
+#pragma unroll
+    for (int i = 0; i < 4; i++) c[i] = b[16 * mt + 4 * g + i];
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) {
+      const gh4 a = *reinterpret_cast<const gh4 *>(W + (16 * mt + r) * 64 + 16 * kt + 4 * g);
+      c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, in[kt], c, 0, 0, 0);
+    }
+    acc[mt] = c;
+  }
+}
+
+__device__ __forceinline__ gh4 gd_relu_h(gf4 v) {
+  gh4 h;
+#pragma unroll
+  for (int i = 0; i < 4; i++) h[i] = (_Float16)fmaxf(v[i], 0.0f);
+  return h;
+}
+
+/* forward pass of one 16-query tile: x = encoded inputs (B layout); out = the
+ * 48 output rows (rows 40..47 are padding); hidden activations optionally
+ * returned (h[layer][kt], fp16, B layout) */
+__device__ __forceinline__ void gd_forward_tile(const _Float16 *__restrict__ Wh, const float *__restrict__ P,
+                                                const gh4 x[4], gf4 out[3], gh4 (*hid)[4]) {
+  gh4 cur[4], nxt[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) cur[k] = x[k];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    gf4 acc[4];
+    const int boff = l == 0 ? GD_B1 : (l == 1 ? GD_B2 : GD_B3);
+    gd_gemm<4>(Wh + GD_H_W(l), P + boff, cur, acc);
+#pragma unroll
+    for (int k = 0; k < 4; k++) nxt[k] = gd_relu_h(acc[k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      cur[k] = nxt[k];
+      if (hid) hid[l][k] = nxt[k];
+    }
+  }
+  gd_gemm<3>(Wh + GD_H_W(3), P + GD_B4, cur, out);
+}
+
+/* the forward pass with weights (W1..W4 stacked row-major, rows padded to
+ * GD_WLD halfs) and biases (b1 b2 b3 b4) staged in LDS */
+#define GD_WLD 72
+__device__ __forceinline__ void gd_forward_tile_lds(const _Float16 *W, const float *b, const gh4 x[4], gf4 out[3]) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  gh4 cur[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) cur[k] = x[k];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const int mts = l == 3 ? 3 : 4;
+    gf4 acc[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      if (mt >= mts) break;
+      gf4 c;
+#pragma unroll
+      for (int i = 0; i < 4; i++) c[i] = b[GD_HID * l + 16 * mt + 4 * g + i];
+#pragma unroll
+      for (int kt = 0; kt < 4; kt++) {
+        const gh4 a = *reinterpret_cast<const gh4 *>(W + (GD_HID * l + 16 * mt + r) * GD_WLD + 16 * kt + 4 * g);
+        c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, cur[kt], c, 0, 0, 0);
+      }
+      acc[mt] = c;
+    }
+    if (l < 3) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) cur[k] = gd_relu_h(acc[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; k++) out[k] = acc[k];
+    }
+  }
+}
+
+/* decoded lobe parameters of one query: mu (unit), kappa, mixture weight */
+struct GdLobes {
+  float mu[GD_LOBES][3];
+  float kappa[GD_LOBES];
+  float w[GD_LOBES];
+};
+
+__device__ __forceinline__ float gd_clamp_logk(float s) { return fminf(fmaxf(s, GD_LOGK_MIN), GD_LOGK_MAX); }
+
+__device__ __forceinline__ void gd_decode(const float o[GD_OUT], GdLobes &L) {
+  float mx = -INFINITY;
+#pragma unroll
+  for (int j = 0; j < GD_LOBES; j++) mx = fmaxf(mx, o[5 * j + 4]);
+  float sum = 0.0f;
+#pragma unroll
+  for (int j = 0; j < GD_LOBES; j++) {
+    const float a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
+    const float len = sqrtf(a * a + b * b + c * c);
+    if (len > 1e-12f) {
+      L.mu[j][0] = a / len;
+      L.mu[j][1] = b / len;
+      L.mu[j][2] = c / len;
+    } else {
+      L.mu[j][0] = 0.0f;
+      L.mu[j][1] = 0.0f;
+      L.mu[j][2] = 1.0f;
+    }
+    L.kappa[j] = expf(gd_clamp_logk(o[5 * j + 3]));
+    L.w[j] = expf(o[5 * j + 4] - mx);
+    sum += L.w[j];
+  }
+#pragma unroll
+  for (int j = 0; j < GD_LOBES; j++) L.w[j] /= sum;
+}
+
+/* normalisation of a vMF lobe on S^2: kappa / (2 pi (1 - exp(-2 kappa))) */
+__device__ __forceinline__ float gd_vmf_norm(float k) { return k / (2.0f * GD_PI * -expm1f(-2.0f * k)); }
+
+/* stored lobes of a query: GD_LSTRIDE floats per lobe = mu xyz, kappa,
+ * mixture weight, weight x normalisation */
+#define GD_LSTRIDE 6
+#define GD_LFLOATS (GD_LOBES * GD_LSTRIDE)
+
+/* q(w) of stored lobes */
+__device__ __forceinline__ float gd_pdf(const float *lob, float dx, float dy, float dz) {
+  float q = 0.0f;
+#pragma unroll
+  for (int j = 0; j < GD_LOBES; j++) {
+    const float *l = lob + GD_LSTRIDE * j;
+    const float c = l[0] * dx + l[1] * dy + l[2] * dz;
+    q += l[5] * expf(l[3] * (c - 1.0f));
+  }
+  return q;
+}
+
+/* draw a direction from the stored lobes with three uniforms: lobe choice,
+ * then Jakob 2012 ("Numerically stable sampling of the von Mises Fisher
+ * distribution on S^2"): cos = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa */
+__device__ __forceinline__ void gd_sample(const float *lob, float u0, float u1, float u2, float &dx, float &dy,
+                                          float &dz) {
+  int j = GD_LOBES - 1;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < GD_LOBES - 1; k++) {
+    acc += lob[GD_LSTRIDE * k + 4];
+    if (u0 < acc) { j = k; break; }
+  }
+  const float *l = lob + GD_LSTRIDE * j;
+  const float k = l[3];
+  float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
+  w = fminf(fmaxf(w, -1.0f), 1.0f);
+  const float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  const float phi = 2.0f * GD_PI * u2;
+  const float vx = s * cosf(phi), vy = s * sinf(phi);
+  /* frame around mu (Duff et al. 2017 branchless orthonormal basis) */
+  const float mx = l[0], my = l[1], mz = l[2];
+  const float sg = copysignf(1.0f, mz);
+  const float a = -1.0f / (sg + mz);
+  const float bb = mx * my * a;
+  const float t1x = 1.0f + sg * mx * mx * a, t1y = sg * bb, t1z = -sg * mx;
+  const float t2x = bb, t2y = sg + my * my * a, t2z = -my;
+  dx = vx * t1x + vy * t2x + w * mx;
+  dy = vx * t1y + vy * t2y + w * my;
+  dz = vx * t1z + vy * t2z + w * mz;
+  const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+  dx /= len;
+  dy /= len;
+  dz /= len;
+}
+
+}  // namespace clv
+#endif

@@ -1,0 +1,488 @@
+/*
+ * guide.hip -- online training of the guiding network (see guide.h).
+ *
+ * After every wavefront pass of a guided render:
+ *   k_guide_count / k_guide_scan / k_guide_scatter
+ *       compact the pass' training records with a radiance sample (L > 0)
+ *       into a list, in slot order (deterministic), and compute their mean
+ *       MC weight F/p;
+ *   k_guide_train (per step)
+ *       one wave per 64 records: forward on MFMA, d(loss)/d(outputs) per
+ *       record, backward on MFMA; every wave writes its summed parameter
+ *       gradient to its own slice of a partial buffer (no atomics);
+ *   k_guide_adam (per step)
+ *       sums the slices in wave order, Adam update of the fp32 master
+ *       parameters, refresh of the fp16 operand copies.
+ * k_guide_eval and the gradient mode of k_guide_adam serve the C ABI's
+ * network probes (clv_guide_eval, clv_guide_grad).
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "guide.h"
+#include "kernels.h"
+
+namespace clv {
+
+#define GT_LD 72     /* LDS row stride (halfs) of activation tiles: 64 + 8 against bank conflicts */
+#define GT_OG 49     /* LDS row stride (floats) of output / gradient rows */
+
+__device__ __forceinline__ bool rec_valid(const GuideRec &r, uint32_t stamp) {
+  return r.stamp == stamp && r.L > 0.0f && isfinite(r.L) && r.pdf > 0.0f && isfinite(r.pdf) && r.t0 > 0.0f;
+}
+
+/* ---------------------------------------------------------- compaction --- */
+__global__ __launch_bounds__(256) void k_guide_count(const GuideRec *rec, uint32_t n, uint32_t stamp,
+                                                     uint32_t *bcount, float *bweight) {
+  __shared__ uint32_t sc[256];
+  __shared__ float sw[256];
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  uint32_t c = 0;
+  float w = 0.0f;
+  if (i < n) {
+    const GuideRec r = rec[i];
+    if (rec_valid(r, stamp)) {
+      c = 1;
+      w = r.L * r.cosv / r.pdf;
+    }
+  }
+  sc[threadIdx.x] = c;
+  sw[threadIdx.x] = w;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sc[threadIdx.x] += sc[threadIdx.x + s];
+      sw[threadIdx.x] += sw[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bcount[blockIdx.x] = sc[0];
+    bweight[blockIdx.x] = sw[0];
+  }
+}
+
+/* one block: exclusive scan of the block counts; totals[0] = valid records,
+ * totals[1] = their mean weight (as float bits) */
+__global__ __launch_bounds__(1024) void k_guide_scan(uint32_t *bcount, const float *bweight, uint32_t nb,
+                                                     uint32_t *totals) {
+  __shared__ uint32_t part[1024];
+  __shared__ double wpart[1024];
+  const uint32_t per = (nb + 1023u) / 1024u;
+  const uint32_t b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+  uint32_t s = 0;
+  double w = 0.0;
+  for (uint32_t b = b0; b < b1; b++) {
+    s += bcount[b];
+    w += (double)bweight[b];
+  }
+  part[threadIdx.x] = s;
+  wpart[threadIdx.x] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    double wacc = 0.0;
+    for (uint32_t t = 0; t < 1024; t++) {
+      uint32_t v = part[t];
+      part[t] = acc;
+      acc += v;
+      wacc += wpart[t];
+    }
+    totals[0] = acc;
+    float mean = acc ? (float)(wacc / (double)acc) : 0.0f;
+    totals[1] = __float_as_uint(mean);
+  }
+  __syncthreads();
+  uint32_t run = part[threadIdx.x];
+  for (uint32_t b = b0; b < b1; b++) {
+    uint32_t v = bcount[b];
+    bcount[b] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_guide_scatter(const GuideRec *rec, uint32_t n, uint32_t stamp,
+                                                       const uint32_t *boffset, uint32_t *list) {
+  __shared__ uint32_t wcount[4];
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const bool v = i < n && rec_valid(rec[i], stamp);
+  const uint64_t m = __ballot(v);
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0) wcount[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t off = boffset[blockIdx.x];
+  for (uint32_t k = 0; k < w; k++) off += wcount[k];
+  if (v) list[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+}
+
+/* ------------------------------------------------------------ training --- */
+/* d(-w log q)/d(outputs) of one record; o = raw outputs, g = 48 gradients */
+__device__ void gd_loss_grad(const float *o, const float d[3], float w, float *g) {
+  for (int k = 0; k < GD_OUT_PAD; k++) g[k] = 0.0f;
+  if (!(w > 0.0f)) return;
+  GdLobes L;
+  gd_decode(o, L);
+  float f[GD_LOBES], c[GD_LOBES], q = 0.0f;
+  for (int j = 0; j < GD_LOBES; j++) {
+    c[j] = L.mu[j][0] * d[0] + L.mu[j][1] * d[1] + L.mu[j][2] * d[2];
+    f[j] = gd_vmf_norm(L.kappa[j]) * expf(L.kappa[j] * (c[j] - 1.0f));
+    q += L.w[j] * f[j];
+  }
+  if (!(q > 1e-30f)) q = 1e-30f;
+  for (int j = 0; j < GD_LOBES; j++) {
+    const float r = L.w[j] * f[j] / q;
+    const float k = L.kappa[j];
+    g[5 * j + 4] = -w * (r - L.w[j]);
+    const float s = o[5 * j + 3];
+    if (s > GD_LOGK_MIN && s < GD_LOGK_MAX) {
+      const float dlogf = 1.0f / k - 2.0f / expm1f(2.0f * k) + (c[j] - 1.0f);
+      g[5 * j + 3] = -w * r * k * dlogf;
+    }
+    const float a = o[5 * j], b = o[5 * j + 1], cc = o[5 * j + 2];
+    const float len = sqrtf(a * a + b * b + cc * cc);
+    if (len > 1e-12f) {
+      const float gm[3] = {-w * r * k * d[0], -w * r * k * d[1], -w * r * k * d[2]};
+      const float mg = L.mu[j][0] * gm[0] + L.mu[j][1] * gm[1] + L.mu[j][2] * gm[2];
+      for (int t = 0; t < 3; t++) g[5 * j + t] = (gm[t] - L.mu[j][t] * mg) / len;
+    }
+  }
+}
+
+/* D^T[16*MT x 16] = A[16*MT x 16*KT] . B[16*KT x 16]: A fp16 row-major with
+ * leading dimension lda (global or LDS), B operands in registers */
+template <int MT, int KT>
+__device__ __forceinline__ void gd_gemm_ab(const _Float16 *A, int lda, const gh4 *b, gf4 *acc) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) {
+    gf4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < KT; kt++) {
+      const gh4 a = *reinterpret_cast<const gh4 *>(A + (16 * mt + r) * lda + 16 * kt + 4 * g);
+      c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b[kt], c, 0, 0, 0);
+    }
+    acc[mt] = c;
+  }
+}
+
+struct GuideTrainArgs {
+  const GuideRec *rec;
+  const uint32_t *list;    /* compacted record slots */
+  const uint32_t *totals;  /* [0] valid records, [1] mean weight (float bits) */
+  uint32_t cap;            /* records used per round */
+  uint32_t step_base;      /* first used record of this step */
+  uint32_t step_len;       /* records per step (grid = step_len / 64) */
+  float wclip;             /* weight cap in units of the mean weight */
+  const _Float16 *Wh;
+  const float *P;
+  float *partial;          /* [wave][GD_NPARAM] */
+};
+
+/* one wave = one block = 64 records */
+__global__ __launch_bounds__(64) void k_guide_train(GuideTrainArgs a) {
+  __shared__ float rin[64][8];
+  __shared__ __attribute__((aligned(16))) _Float16 H[4][64 * GT_LD]; /* activations A0..A3, [feature][record] */
+  __shared__ __attribute__((aligned(16))) _Float16 DZ[64 * GT_LD];   /* back-propagated rows, [neuron][record] */
+  __shared__ float OG[64 * GT_OG];        /* outputs, then their gradients, [record][48] */
+  const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+  const uint32_t nvalid = a.totals[0];
+  const float wmean = __uint_as_float(a.totals[1]);
+  const uint32_t nuse = min(nvalid, a.cap);
+  const uint32_t u = a.step_base + blockIdx.x * 64u + (uint32_t)lane;
+  float wgt = 0.0f, dir[3] = {0.0f, 0.0f, 1.0f};
+  {
+    float in[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (u < nuse && wmean > 0.0f) {
+      const uint32_t slot = a.list[(uint32_t)(((uint64_t)u * nvalid) / nuse)];
+      const GuideRec R = a.rec[slot];
+      wgt = fminf(R.L * R.cosv / R.pdf / wmean, a.wclip);
+      for (int k = 0; k < 3; k++) {
+        in[k] = R.p[k];
+        in[3 + k] = R.n[k];
+        dir[k] = R.d[k];
+      }
+      in[6] = R.lam;
+    }
+    for (int k = 0; k < 8; k++) rin[lane][k] = in[k];
+  }
+  __syncthreads();
+  /* forward, one 16-record tile at a time; every activation to LDS */
+  for (int nt = 0; nt < 4; nt++) {
+    const int q = 16 * nt + r;
+    const float p[3] = {rin[q][0], rin[q][1], rin[q][2]}, n[3] = {rin[q][3], rin[q][4], rin[q][5]};
+    gh4 x[4], hid[3][4];
+    gf4 out[3];
+    gd_encode(p, n, rin[q][6], x);
+    gd_forward_tile(a.Wh, a.P, x, out, hid);
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int f = 16 * kt + 4 * g + i;
+        H[0][f * GT_LD + q] = x[kt][i];
+        H[1][f * GT_LD + q] = hid[0][kt][i];
+        H[2][f * GT_LD + q] = hid[1][kt][i];
+        H[3][f * GT_LD + q] = hid[2][kt][i];
+      }
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) OG[q * GT_OG + 16 * mt + 4 * g + i] = out[mt][i];
+  }
+  __syncthreads();
+  /* output gradient of this lane's record */
+  {
+    float o[GD_OUT_PAD], gr[GD_OUT_PAD];
+    for (int k = 0; k < GD_OUT_PAD; k++) o[k] = OG[lane * GT_OG + k];
+    gd_loss_grad(o, dir, wgt, gr);
+    for (int k = 0; k < GD_OUT_PAD; k++) OG[lane * GT_OG + k] = gr[k];
+  }
+  __syncthreads();
+  float *part = a.partial + (size_t)blockIdx.x * GD_NPARAM;
+  /* layer 4: D = output gradients (48 rows) */
+  gh4 din[4][4]; /* [record tile][k-step]: B operands of the current layer's D */
+  for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+    for (int kt = 0; kt < 3; kt++) {
+      gh4 v;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float gv = OG[(16 * nt + r) * GT_OG + 16 * kt + 4 * g + i];
+        v[i] = (_Float16)gv;
+        DZ[(16 * kt + 4 * g + i) * GT_LD + 16 * nt + r] = (_Float16)gv;
+      }
+      din[nt][kt] = v;
+    }
+  if (lane < GD_OUT_PAD) {
+    float s = 0.0f;
+    for (int q = 0; q < 64; q++) s += OG[q * GT_OG + lane];
+    part[GD_B4 + lane] = s;
+  }
+  __syncthreads();
+  for (int layer = 4; layer >= 1; layer--) {
+    const int rows = layer == 4 ? GD_OUT_PAD : GD_HID;
+    const int wofs = layer == 1 ? GD_W1 : (layer == 2 ? GD_W2 : (layer == 3 ? GD_W3 : GD_W4));
+    /* dW = D . A_{layer-1}^T : M = rows, N = 64 features, K = 64 records */
+    for (int mt = 0; mt < rows / 16; mt++)
+      for (int nt = 0; nt < 4; nt++) {
+        gf4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) {
+          const gh4 av = *reinterpret_cast<const gh4 *>(&DZ[(16 * mt + r) * GT_LD + 16 * kt + 4 * g]);
+          const gh4 bv = *reinterpret_cast<const gh4 *>(&H[layer - 1][(16 * nt + r) * GT_LD + 16 * kt + 4 * g]);
+          c = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) part[wofs + (16 * mt + 4 * g + i) * 64 + 16 * nt + r] = c[i];
+      }
+    if (layer == 1) break;
+    /* D_{layer-1} = (W^T . D) masked by A_{layer-1} > 0 */
+    gh4 dnew[4][4];
+    const _Float16 *WT = a.Wh + GD_H_WT(layer - 1);
+    for (int nt = 0; nt < 4; nt++) {
+      gf4 acc[4];
+      if (rows == GD_OUT_PAD) gd_gemm_ab<4, 3>(WT, GD_OUT_PAD, din[nt], acc);
+      else gd_gemm_ab<4, 4>(WT, GD_HID, din[nt], acc);
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        gh4 v;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int f = 16 * mt + 4 * g + i;
+          const bool on = (float)H[layer - 1][f * GT_LD + 16 * nt + r] > 0.0f;
+          v[i] = on ? (_Float16)acc[mt][i] : (_Float16)0.0f;
+        }
+        dnew[nt][mt] = v;
+      }
+    }
+    __syncthreads(); /* every lane is done reading DZ */
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) DZ[(16 * mt + 4 * g + i) * GT_LD + 16 * nt + r] = dnew[nt][mt][i];
+        din[nt][mt] = dnew[nt][mt];
+      }
+    __syncthreads();
+    {
+      const int bofs = layer == 2 ? GD_B1 : (layer == 3 ? GD_B2 : GD_B3);
+      float s = 0.0f;
+      for (int q = 0; q < 64; q++) s += (float)DZ[lane * GT_LD + q];
+      part[bofs + lane] = s;
+    }
+  }
+}
+
+/* first stage of the gradient sum: chunk c of GD_RCHUNK wave slices per
+ * parameter (grid.y = chunks), in wave order */
+#define GD_RCHUNK 16u
+__global__ __launch_bounds__(256) void k_guide_reduce(const float *partial, const uint32_t *totals, uint32_t cap,
+                                                      uint32_t step_base, uint32_t step_len, float *chunks) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= GD_NPARAM) return;
+  const uint32_t nuse = min(totals[0], cap);
+  if (step_base >= nuse) return;
+  const uint32_t waves = (min(step_len, nuse - step_base) + 63u) / 64u;
+  const uint32_t w0 = blockIdx.y * GD_RCHUNK, w1 = min(waves, w0 + GD_RCHUNK);
+  float s = 0.0f;
+  for (uint32_t w = w0; w < w1; w++) s += partial[(size_t)w * GD_NPARAM + p];
+  chunks[(size_t)blockIdx.y * GD_NPARAM + p] = s;
+}
+
+struct GuideAdamArgs {
+  float *P;
+  float *m;
+  float *v;
+  _Float16 *Wh;
+  const float *chunks;      /* first-stage sums, [chunk][GD_NPARAM] */
+  const uint32_t *totals;
+  uint32_t cap, step_base, step_len;
+  float lr, beta1, beta2, eps;
+  float bc1, bc2;           /* 1 - beta^t */
+  float *grad_out;          /* gradient mode: write the mean gradient here, no update */
+};
+
+__device__ __forceinline__ void refresh_half(_Float16 *Wh, uint32_t p, float val) {
+  int layer, row, col, rows;
+  uint32_t base;
+  if (p < GD_B1) { layer = 0; base = GD_W1; rows = GD_HID; }
+  else if (p >= GD_W2 && p < GD_B2) { layer = 1; base = GD_W2; rows = GD_HID; }
+  else if (p >= GD_W3 && p < GD_B3) { layer = 2; base = GD_W3; rows = GD_HID; }
+  else if (p >= GD_W4 && p < GD_B4) { layer = 3; base = GD_W4; rows = GD_OUT_PAD; }
+  else return;
+  row = (int)((p - base) / 64u);
+  col = (int)((p - base) % 64u);
+  Wh[GD_H_W(layer) + row * 64 + col] = (_Float16)val;
+  Wh[GD_H_WT(layer) + col * rows + row] = (_Float16)val;
+}
+
+__global__ __launch_bounds__(256) void k_guide_adam(GuideAdamArgs a) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= GD_NPARAM) return;
+  const uint32_t nuse = min(a.totals[0], a.cap);
+  if (a.step_base >= nuse) return; /* an empty step: no update */
+  const uint32_t len = min(a.step_len, nuse - a.step_base);
+  const uint32_t nchunk = ((len + 63u) / 64u + GD_RCHUNK - 1u) / GD_RCHUNK;
+  float gsum = 0.0f;
+  for (uint32_t c = 0; c < nchunk; c++) gsum += a.chunks[(size_t)c * GD_NPARAM + p];
+  const float gr = gsum / (float)len;
+  if (a.grad_out) {
+    a.grad_out[p] = gr;
+    return;
+  }
+  const float m = a.beta1 * a.m[p] + (1.0f - a.beta1) * gr;
+  const float v = a.beta2 * a.v[p] + (1.0f - a.beta2) * gr * gr;
+  a.m[p] = m;
+  a.v[p] = v;
+  const float val = a.P[p] - a.lr * (m / a.bc1) / (sqrtf(v / a.bc2) + a.eps);
+  a.P[p] = val;
+  refresh_half(a.Wh, p, val);
+}
+
+/* fp16 operand copies from the master parameters */
+__global__ __launch_bounds__(256) void k_guide_halves(const float *P, _Float16 *Wh) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p < GD_NPARAM) refresh_half(Wh, p, P[p]);
+}
+
+/* network probe: raw outputs (40) and decoded lobes (40) of n inputs
+ * (p xyz, n xyz, lambda; 7 floats each) */
+__global__ __launch_bounds__(64) void k_guide_eval(const float *in, uint32_t n, const _Float16 *Wh, const float *P,
+                                                   float *raw, float *lobes) {
+  __shared__ float rin[64][8];
+  __shared__ float OG[64 * GT_OG];
+  const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+  const uint32_t i = blockIdx.x * 64u + (uint32_t)lane;
+  for (int k = 0; k < 7; k++) rin[lane][k] = i < n ? in[7 * (size_t)i + k] : 0.0f;
+  __syncthreads();
+  for (int nt = 0; nt < 4; nt++) {
+    const int q = 16 * nt + r;
+    const float p[3] = {rin[q][0], rin[q][1], rin[q][2]}, nn[3] = {rin[q][3], rin[q][4], rin[q][5]};
+    gh4 x[4];
+    gf4 out[3];
+    gd_encode(p, nn, rin[q][6], x);
+    gd_forward_tile(Wh, P, x, out, nullptr);
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) OG[q * GT_OG + 16 * mt + 4 * g + k] = out[mt][k];
+  }
+  __syncthreads();
+  if (i >= n) return;
+  float o[GD_OUT];
+  for (int k = 0; k < GD_OUT; k++) {
+    o[k] = OG[lane * GT_OG + k];
+    raw[GD_OUT * (size_t)i + k] = o[k];
+  }
+  GdLobes L;
+  gd_decode(o, L);
+  for (int j = 0; j < GD_LOBES; j++) {
+    float *d = lobes + GD_OUT * (size_t)i + 5 * j;
+    d[0] = L.mu[j][0];
+    d[1] = L.mu[j][1];
+    d[2] = L.mu[j][2];
+    d[3] = L.kappa[j];
+    d[4] = L.w[j];
+  }
+}
+
+/* ---------------------------------------------------------- launchers --- */
+void launch_guide_collect(hipStream_t st, const GuideRec *rec, uint32_t n, uint32_t stamp, uint32_t *bcount,
+                          float *bweight, uint32_t *totals, uint32_t *list) {
+  const uint32_t nb = (n + 255u) / 256u;
+  hipLaunchKernelGGL(k_guide_count, dim3(nb), dim3(256), 0, st, rec, n, stamp, bcount, bweight);
+  hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(1024), 0, st, bcount, bweight, nb, totals);
+  hipLaunchKernelGGL(k_guide_scatter, dim3(nb), dim3(256), 0, st, rec, n, stamp, bcount, list);
+}
+
+void launch_guide_step(hipStream_t st, const GuideRec *rec, const uint32_t *list, const uint32_t *totals,
+                       uint32_t cap, uint32_t step_base, uint32_t step_len, float wclip, float *P, float *m,
+                       float *v, _Float16 *Wh, float *partial, float lr, uint32_t t, float *grad_out) {
+  GuideTrainArgs ta;
+  ta.rec = rec;
+  ta.list = list;
+  ta.totals = totals;
+  ta.cap = cap;
+  ta.step_base = step_base;
+  ta.step_len = step_len;
+  ta.wclip = wclip;
+  ta.Wh = Wh;
+  ta.P = P;
+  ta.partial = partial;
+  const uint32_t waves = (step_len + 63u) / 64u;
+  hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), 0, st, ta);
+  /* chunk sums go behind the wave slices of the partial buffer */
+  float *chunks = partial + (size_t)waves * GD_NPARAM;
+  const uint32_t nchunk = (waves + GD_RCHUNK - 1u) / GD_RCHUNK;
+  hipLaunchKernelGGL(k_guide_reduce, dim3((GD_NPARAM + 255u) / 256u, nchunk), dim3(256), 0, st, partial, totals, cap,
+                     step_base, step_len, chunks);
+  GuideAdamArgs aa;
+  aa.P = P;
+  aa.m = m;
+  aa.v = v;
+  aa.Wh = Wh;
+  aa.chunks = chunks;
+  aa.totals = totals;
+  aa.cap = cap;
+  aa.step_base = step_base;
+  aa.step_len = step_len;
+  aa.lr = lr;
+  aa.beta1 = 0.9f;
+  aa.beta2 = 0.99f;
+  aa.eps = 1e-8f;
+  aa.bc1 = 1.0f - powf(0.9f, (float)t);
+  aa.bc2 = 1.0f - powf(0.99f, (float)t);
+  aa.grad_out = grad_out;
+  hipLaunchKernelGGL(k_guide_adam, dim3((GD_NPARAM + 255u) / 256u), dim3(256), 0, st, aa);
+}
+
+void launch_guide_halves(hipStream_t st, const float *P, _Float16 *Wh) {
+  hipLaunchKernelGGL(k_guide_halves, dim3((GD_NPARAM + 255u) / 256u), dim3(256), 0, st, P, Wh);
+}
+
+void launch_guide_eval(hipStream_t st, const float *in, uint32_t n, const _Float16 *Wh, const float *P, float *raw,
+                       float *lobes) {
+  hipLaunchKernelGGL(k_guide_eval, dim3((n + 63u) / 64u), dim3(64), 0, st, in, n, Wh, P, raw, lobes);
+}
+
+}  // namespace clv

@@ -7,6 +7,7 @@
 
 #include "../../include/clovers_hip.h"
 #include "clv_layout.h"
+#include "guide.h"
 
 namespace clv {
 
@@ -38,8 +39,8 @@ struct DevJob {
   uint32_t pixel_base;    /* first accumulator pixel of the job */
   uint32_t spp;           /* samples per pixel of the whole job */
   uint32_t fp64;          /* accumulate in f64 */
-  uint32_t guiding;
-  uint32_t pad[2];
+  uint32_t guiding;        /* clv_guiding */
+  float lo[3], hi[3];      /* scene box: guiding inputs are positions normalised to it */
 };
 
 /* one job's share of a wavefront pass: `samples` samples of every pixel */
@@ -82,6 +83,19 @@ struct PathBuf {
   uint32_t *meta;   /* META_*: hero wavelength index, depth, job */
   float *hit_t;
   uint32_t *hit_p;  /* primitive record offset of the closest hit, ~0u on a miss */
+  uint32_t *gcnt;   /* guided renders: diffuse vertices recorded per path this pass (else NULL) */
+};
+
+/* guided sampling state of a pass */
+struct GuideArgs {
+  float *lobes;            /* [path][GD_LFLOATS]: decoded lobes of the path's current diffuse vertex */
+  GuideRec *rec;           /* [path][GD_SLOTS] training records */
+  const _Float16 *Wh;      /* fp16 operand copies of the network */
+  const float *P;          /* fp32 parameters (biases) */
+  float alpha;             /* probability of drawing from the lobes; 0 = plain sampling */
+  uint32_t stamp;          /* pass id written into records */
+  uint32_t record;         /* write training records */
+  uint32_t on;             /* the batch has guided jobs: lobes/rec/gcnt are valid */
 };
 
 struct ShadeQueues {
@@ -152,7 +166,19 @@ void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const 
 void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                     int variant);
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
-                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap);
+                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga);
+/* learned-proposal lobes for the Lambertian queue of the current bounce */
+void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GuideArgs &ga,
+                        const uint32_t *queue, const uint32_t *count, uint32_t max_paths);
+/* guide.hip: record compaction, training steps, network probes */
+void launch_guide_collect(hipStream_t st, const GuideRec *rec, uint32_t n, uint32_t stamp, uint32_t *bcount,
+                          float *bweight, uint32_t *totals, uint32_t *list);
+void launch_guide_step(hipStream_t st, const GuideRec *rec, const uint32_t *list, const uint32_t *totals,
+                       uint32_t cap, uint32_t step_base, uint32_t step_len, float wclip, float *P, float *m,
+                       float *v, _Float16 *Wh, float *partial, float lr, uint32_t t, float *grad_out);
+void launch_guide_halves(hipStream_t st, const float *P, _Float16 *Wh);
+void launch_guide_eval(hipStream_t st, const float *in, uint32_t n, const _Float16 *Wh, const float *P, float *raw,
+                       float *lobes);
 void launch_resolve(hipStream_t st, const PathBuf &pb, const ResolveParams &rp, const float4 *cie);
 void launch_finalize(hipStream_t st, const ResolveParams &rp, float *out, float *out_sq);
 void launch_zero_accum(hipStream_t st, const ResolveParams &rp);

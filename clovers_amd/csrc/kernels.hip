@@ -427,6 +427,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
     pb.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pb.rng[i] = s;
     pb.meta[i] = wl | (it.job << META_JOB_SHIFT); /* depth 0 */
+    if (pb.gcnt) pb.gcnt[i] = 0u;
     qout[i] = i;
   }
 }
@@ -940,9 +941,22 @@ __device__ __forceinline__ void rotate_wl(uint32_t hero_idx, int wl[4]) {
   for (int j = 0; j < 4; j++) wl[j] = CLV_LAMBDA_MIN + (int)((hero_idx + 100u * (uint32_t)j) % 400u);
 }
 
+/* a radiance sample c0 (hero wavelength, already times the path throughput)
+ * reached the path: credit it to the path's open training records, each as
+ * c0 / (throughput right after that vertex) = radiance along its direction */
+__device__ __forceinline__ void gd_credit(const PathBuf &pb, const GuideArgs &ga, uint32_t i, float c0) {
+  if (!(c0 > 0.0f)) return;
+  const uint32_t cnt = pb.gcnt[i];
+  const uint32_t m = cnt < GD_SLOTS ? cnt : GD_SLOTS;
+  for (uint32_t k = 0; k < m; k++) {
+    GuideRec &r = ga.rec[(size_t)i * GD_SLOTS + ((cnt - 1u - k) % GD_SLOTS)];
+    if (r.t0 > 0.0f) r.L += c0 / r.t0;
+  }
+}
+
 /* One shading step of one path; returns true if the path continues. */
-template <uint32_t CLS>
-__device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint32_t i) {
+template <uint32_t CLS, bool GUIDE>
+__device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint32_t i, const GuideArgs &ga) {
   uint32_t meta = pb.meta[i];
   uint32_t hero_idx = meta & META_HERO_MASK;
   uint32_t depth = (meta >> META_DEPTH_SHIFT) & META_DEPTH_MASK;
@@ -958,6 +972,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     L.z += T.z * bg[wl[2] - CLV_LAMBDA_MIN];
     L.w += T.w * bg[wl[3] - CLV_LAMBDA_MIN];
     pb.rad[i] = L;
+    if (GUIDE && ga.record && jb.guiding) gd_credit(pb, ga, i, T.x * bg[wl[0] - CLV_LAMBDA_MIN]);
     return false;
   }
   float4 o4 = pb.ray_o[i], d4 = pb.ray_d[i];
@@ -974,6 +989,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
       L.z += T.z * e[wl[2] - CLV_LAMBDA_MIN];
       L.w += T.w * e[wl[3] - CLV_LAMBDA_MIN];
       pb.rad[i] = L;
+      if (GUIDE && ga.record && jb.guiding) gd_credit(pb, ga, i, T.x * e[wl[0] - CLV_LAMBDA_MIN]);
     }
     return false;
   }
@@ -995,9 +1011,19 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
   if (CLS == MC_LAMBERTIAN) {
     /* MixturePDF(HitablePDF(lights), CosinePDF(normal)) -- trace.rs:80-120, pdf.rs:106-141 */
     Onb b = onb_from_w(h.n);
+    /* guided: the light-tree half of the mixture stays; the cosine half draws
+     * from the learned lobes with probability alpha (one-sample MIS over all
+     * three densities).  Unguided jobs draw exactly the plain sequence. */
+    const bool guided = GUIDE && ga.on && jb.guiding && ga.alpha > 0.0f;
+    const float *lob = guided ? ga.lobes + (size_t)i * GD_LFLOATS : nullptr;
     V gen;
     if (coin(s)) {
       gen = tree_sample(sc, jb, h.p, s);
+    } else if (guided && rnd(s) < ga.alpha) {
+      float gx, gy, gz;
+      const float u0 = rnd(s), u1 = rnd(s), u2 = rnd(s);
+      gd_sample(lob, u0, u1, u2, gx, gy, gz);
+      gen = mk(gx, gy, gz);
     } else {
       float r1 = rnd(s), r2 = rnd(s);
       float z = sqrtf(1.0f - r2);
@@ -1009,6 +1035,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     V ndn = unit(nd);
     float cs = dot(ndn, b.w);
     float cp = cs <= 0.0f ? 0.0f : cs / PI_F;
+    if (guided) cp = ga.alpha * gd_pdf(lob, ndn.x, ndn.y, ndn.z) + (1.0f - ga.alpha) * cp;
     float mis = 0.5f * lp + 0.5f * cp;
     float c2 = dot(h.n, ndn);
     if (c2 < 0.0f) { /* scattering_pdf None: return emitted (0 for Lambertian) */
@@ -1020,6 +1047,28 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     T.y = T.y * att[1] * spdf / mis;
     T.z = T.z * att[2] * spdf / mis;
     T.w = T.w * att[3] * spdf / mis;
+    if (GUIDE && ga.record && jb.guiding && depth + 1u <= jb.max_depth) {
+      const uint32_t cnt = pb.gcnt[i];
+      GuideRec r;
+      r.p[0] = 2.0f * (h.p.x - jb.lo[0]) / (jb.hi[0] - jb.lo[0]) - 1.0f;
+      r.p[1] = 2.0f * (h.p.y - jb.lo[1]) / (jb.hi[1] - jb.lo[1]) - 1.0f;
+      r.p[2] = 2.0f * (h.p.z - jb.lo[2]) / (jb.hi[2] - jb.lo[2]) - 1.0f;
+      r.n[0] = h.n.x;
+      r.n[1] = h.n.y;
+      r.n[2] = h.n.z;
+      r.d[0] = ndn.x;
+      r.d[1] = ndn.y;
+      r.d[2] = ndn.z;
+      r.lam = (float)hero_idx / 400.0f;
+      r.pdf = mis;
+      r.cosv = c2;
+      r.t0 = T.x;
+      r.L = 0.0f;
+      r.stamp = ga.stamp;
+      r.pad = 0u;
+      ga.rec[(size_t)i * GD_SLOTS + cnt % GD_SLOTS] = r;
+      pb.gcnt[i] = cnt + 1u;
+    }
   } else {
     if (CLS == MC_METAL) { /* metal.rs:26-52 */
       V r = reflect(d, h.n);
@@ -1056,8 +1105,9 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
   return true;
 }
 
+template <bool GUIDE>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQueues sq, uint32_t *qout,
-                                                uint32_t *counts, uint32_t out_slot) {
+                                                uint32_t *counts, uint32_t out_slot, GuideArgs ga) {
   __shared__ uint32_t agg[1 * 4 + 1];
   uint32_t c[MC_COUNT], chunks[MC_COUNT + 1];
   chunks[0] = 0;
@@ -1084,15 +1134,120 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
       if (idx < c[cls]) {
         path = sq.q[cls][idx];
         switch (cls) {
-          case MC_MISS: cont = shade_path<MC_MISS>(sc, pb, path); break;
-          case MC_LIGHT: cont = shade_path<MC_LIGHT>(sc, pb, path); break;
-          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN>(sc, pb, path); break;
-          case MC_METAL: cont = shade_path<MC_METAL>(sc, pb, path); break;
-          default: cont = shade_path<MC_GLASS>(sc, pb, path); break;
+          case MC_MISS: cont = shade_path<MC_MISS, GUIDE>(sc, pb, path, ga); break;
+          case MC_LIGHT: cont = shade_path<MC_LIGHT, GUIDE>(sc, pb, path, ga); break;
+          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN, GUIDE>(sc, pb, path, ga); break;
+          case MC_METAL: cont = shade_path<MC_METAL, GUIDE>(sc, pb, path, ga); break;
+          default: cont = shade_path<MC_GLASS, GUIDE>(sc, pb, path, ga); break;
         }
       }
     }
     block_append<1, 256>(cont ? 0u : 1u, path, out_q, counts + out_slot, agg);
+  }
+}
+
+/* Learned-proposal lobes for every guided path of the Lambertian queue.
+ * Resident blocks of 4 waves stage the fp16 weights and the biases in LDS
+ * once, then loop over the queue 256 paths at a time: one lane per path
+ * reconstructs the hit (position normalised to the job's scene box, oriented
+ * normal, hero wavelength); each wave runs the MLP on MFMA for its 64 paths,
+ * 16 at a time, and 4 lanes per path decode 2 lobes each straight to the
+ * lobe buffer.  The loop is block-uniform, so LDS exchanges use __syncthreads. */
+#define GI_OG 52 /* LDS row stride (floats) of one tile's outputs */
+__global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, GuideArgs ga, const uint32_t *queue,
+                                                     const uint32_t *count) {
+  /* W1..W4 row-major with rows padded to GD_WLD halfs (bank-conflict-free A operands) */
+  __shared__ __attribute__((aligned(16))) _Float16 wl[(3 * GD_HID + GD_OUT_PAD) * GD_WLD];
+  __shared__ float bl[GD_HID * 3 + GD_OUT_PAD];
+  __shared__ float rin[4][64][8];
+  __shared__ float og[4][16 * GI_OG];
+  {
+    /* 8 uint4 per 64-half row; row i of the stacked W1..W4 goes to LDS row i */
+    const uint4 *src = reinterpret_cast<const uint4 *>(ga.Wh);
+    for (uint32_t k = threadIdx.x; k < GD_H_WT(0) / 8u; k += 256u)
+      *reinterpret_cast<uint4 *>(wl + (k >> 3) * GD_WLD + (k & 7u) * 8u) = src[k];
+    for (uint32_t k = threadIdx.x; k < GD_HID * 3u + GD_OUT_PAD; k += 256u) {
+      const uint32_t l = k / GD_HID;
+      bl[k] = ga.P[l == 0 ? GD_B1 + k : (l == 1 ? GD_B2 + k - GD_HID : (l == 2 ? GD_B3 + k - 2 * GD_HID
+                                                                                 : GD_B4 + k - 3 * GD_HID))];
+    }
+  }
+  const uint32_t n = *count;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const uint32_t qi = base + threadIdx.x;
+    uint32_t path = 0xFFFFFFFFu;
+    float in[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (qi < n) {
+      const uint32_t pth = queue[qi];
+      const uint32_t meta = pb.meta[pth];
+      const DevJob &jb = sc.jobs[meta >> META_JOB_SHIFT];
+      if (jb.guiding) {
+        path = pth;
+        const float4 o4 = pb.ray_o[pth], d4 = pb.ray_d[pth];
+        const Hit h = hit_record(sc, pb.hit_p[pth], xyz(o4), xyz(d4), pb.hit_t[pth]);
+        in[0] = 2.0f * (h.p.x - jb.lo[0]) / (jb.hi[0] - jb.lo[0]) - 1.0f;
+        in[1] = 2.0f * (h.p.y - jb.lo[1]) / (jb.hi[1] - jb.lo[1]) - 1.0f;
+        in[2] = 2.0f * (h.p.z - jb.lo[2]) / (jb.hi[2] - jb.lo[2]) - 1.0f;
+        in[3] = h.n.x;
+        in[4] = h.n.y;
+        in[5] = h.n.z;
+        in[6] = (float)(meta & META_HERO_MASK) / 400.0f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) rin[w][lane][k] = in[k];
+    rin[w][lane][7] = __uint_as_float(path);
+    __syncthreads();
+#pragma unroll 1
+    for (int nt = 0; nt < 4; nt++) {
+      const int q = 16 * nt + r;
+      {
+        const float p[3] = {rin[w][q][0], rin[w][q][1], rin[w][q][2]};
+        const float nn[3] = {rin[w][q][3], rin[w][q][4], rin[w][q][5]};
+        gh4 x[4];
+        gf4 out[3];
+        gd_encode(p, nn, rin[w][q][6], x);
+        gd_forward_tile_lds(wl, bl, x, out);
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+          for (int k = 0; k < 4; k++) og[w][r * GI_OG + 16 * mt + 4 * g + k] = out[mt][k];
+      }
+      __syncthreads();
+      const uint32_t pth = __float_as_uint(rin[w][q][7]);
+      if (pth != 0xFFFFFFFFu) {
+        /* lane g decodes lobes 2g, 2g+1 of path pth; the softmax needs all 8 logits */
+        const float *o = &og[w][r * GI_OG];
+        float mx = -INFINITY;
+#pragma unroll
+        for (int j = 0; j < GD_LOBES; j++) mx = fmaxf(mx, o[5 * j + 4]);
+        float sum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < GD_LOBES; j++) sum += expf(o[5 * j + 4] - mx);
+        float v[2 * GD_LSTRIDE];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int j = 2 * g + h;
+          const float a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
+          const float len = sqrtf(a * a + b * b + c * c);
+          const bool ok = len > 1e-12f;
+          const float k = expf(gd_clamp_logk(o[5 * j + 3]));
+          const float wj = expf(o[5 * j + 4] - mx) / sum;
+          v[GD_LSTRIDE * h + 0] = ok ? a / len : 0.0f;
+          v[GD_LSTRIDE * h + 1] = ok ? b / len : 0.0f;
+          v[GD_LSTRIDE * h + 2] = ok ? c / len : 1.0f;
+          v[GD_LSTRIDE * h + 3] = k;
+          v[GD_LSTRIDE * h + 4] = wj;
+          v[GD_LSTRIDE * h + 5] = wj * gd_vmf_norm(k);
+        }
+        float4 *dst = reinterpret_cast<float4 *>(ga.lobes + (size_t)pth * GD_LFLOATS + 2 * GD_LSTRIDE * g);
+        dst[0] = make_float4(v[0], v[1], v[2], v[3]);
+        dst[1] = make_float4(v[4], v[5], v[6], v[7]);
+        dst[2] = make_float4(v[8], v[9], v[10], v[11]);
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -1254,10 +1409,26 @@ void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const
     hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
 }
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
-                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap) {
+                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga) {
   uint32_t g = grid_for(max_paths + 64u * MC_COUNT, 256, grid_cap);
-  hipLaunchKernelGGL(k_shade, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot);
+  if (ga.on) hipLaunchKernelGGL(k_shade<true>, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+  else hipLaunchKernelGGL(k_shade<false>, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
 }
+void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GuideArgs &ga,
+                        const uint32_t *queue, const uint32_t *count, uint32_t max_paths) {
+  /* resident blocks only: each stages the weights once */
+  static uint32_t resident = 0;
+  if (!resident) {
+    int per_cu = 0, dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_guide_infer, 256, 0);
+    resident = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+  }
+  hipLaunchKernelGGL(k_guide_infer, dim3(std::min(resident, grid_for(max_paths, 256, 1u << 30))), dim3(256), 0, st,
+                     sc, pb, ga, queue, count);
+}
+
 void launch_resolve(hipStream_t st, const PathBuf &pb, const ResolveParams &rp, const float4 *cie) {
   hipLaunchKernelGGL(k_resolve, dim3(grid_for(rp.threads, 256, 65535)), dim3(256), 0, st, pb, rp, cie);
 }

@@ -160,6 +160,8 @@ typedef struct {
 
 #define CLV_FLAG_TIME_KERNELS 1u  /* time every traversal launch with HIP events (clv_counters.extend_ms) */
 #define CLV_FLAG_COUNT 2u         /* counting traversal: node visits / primitive tests (slower) */
+#define CLV_FLAG_GUIDE_FROZEN 4u  /* guided jobs sample with the network but never train it (ablation) */
+#define CLV_FLAG_GUIDE_KEEP 8u    /* guided jobs start from the context's current network instead of a fresh one */
 
 /* Counters of the last clv_render / clv_primary_hits on a context. */
 typedef struct {
@@ -256,7 +258,15 @@ typedef struct {
   uint32_t trace_kernel;    /* clv_trace_kernel */
   uint32_t lds_bytes;       /* dynamic LDS per block of CLV_TRACE_LDS_CACHED (<= 64512) */
   uint32_t shade_grid_cap;  /* largest grid of the shading kernel, in 256-thread blocks */
-  uint32_t _pad[4];
+  uint32_t guide_train_spp; /* guided jobs: samples per pixel between training rounds (default 4) */
+  float guide_alpha;        /* at a diffuse vertex, share of the cosine half of the mixture drawn from the
+                               learned lobes (default 0.9) */
+  float guide_lr;           /* Adam learning rate (default 0.005) */
+  uint32_t guide_records;   /* training records used per round (default 262144, in steps of 16384) */
+  float guide_wclip;        /* training weight F/p capped at this multiple of its mean (default 1e6) */
+  uint32_t guide_rounds;    /* training rounds per guided render (default 16); later samples use the
+                               trained network in large passes without training */
+  uint32_t _pad[2];
 } clv_tuning;
 clv_status clv_tuning_get(const clv_ctx *ctx, clv_tuning *out);
 clv_status clv_tuning_set(clv_ctx *ctx, const clv_tuning *t);
@@ -267,6 +277,29 @@ clv_status clv_tuning_set(clv_ctx *ctx, const clv_tuning *t);
  * it.  Outputs of the trials are discarded. */
 clv_status clv_autotune(clv_ctx *ctx, const clv_job *jobs, uint32_t n_jobs, uint64_t trial_samples,
                         clv_tuning *best_out);
+
+/* Guiding network probes (no reference counterpart: the learned proposal is
+ * this library's own).  The network maps (position normalised to the scene
+ * box, oriented normal, (lambda - 380) / 400) to 8 vMF lobes; its fp32
+ * parameter vector has CLV_GUIDE_NPARAM entries (W1 b1 W2 b2 W3 b3 W4 b4,
+ * W row-major [out][in]: 64x64, 64x64, 64x64, 48x64 with output rows 40..47
+ * unused).  A guided render starts from a fresh network seeded by the job's
+ * seed unless CLV_FLAG_GUIDE_KEEP is set. */
+#define CLV_GUIDE_NPARAM 15600u
+#define CLV_GUIDE_OUT 40u
+/* fresh parameters from `seed` */
+clv_status clv_guide_reset(clv_ctx *ctx, uint64_t seed);
+clv_status clv_guide_params(clv_ctx *ctx, float *params_out, uint32_t n);
+clv_status clv_guide_set_params(clv_ctx *ctx, const float *params, uint32_t n);
+/* forward pass on the device: inputs n x 7 (p xyz, normal xyz, lambda);
+ * raw_out n x 40 network outputs; lobes_out n x 40 (per lobe mu xyz, kappa,
+ * mixture weight).  Either output may be NULL. */
+clv_status clv_guide_eval(clv_ctx *ctx, const float *inputs, uint32_t n, float *raw_out, float *lobes_out);
+/* gradient of the training loss on the device, without an update: samples
+ * n x 11 (p xyz, normal xyz, lambda, direction xyz, weight F/p > 0); loss =
+ * mean over samples of -(w / mean w) log q(direction); grad_out receives
+ * CLV_GUIDE_NPARAM floats. */
+clv_status clv_guide_grad(clv_ctx *ctx, const float *samples, uint32_t n, float *grad_out);
 
 /* Primary-ray closest hits for sample `opts->sample_offset` of every pixel,
  * same camera and ray seeds as clv_render (the parity probe of the hit path).

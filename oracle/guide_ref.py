@@ -1,0 +1,206 @@
+"""ORACLE -- test infrastructure only.
+
+float64 numpy restatement of the guiding network of the HIP library
+(clovers_amd/csrc/guide.h, guide.hip): input encoding, the 3x64 ReLU MLP,
+the 8-lobe von Mises-Fisher mixture decoded from its 40 outputs, the mixture
+density and its sampling, and the training loss with its gradient by manual
+back-propagation.  The learned proposal has no counterpart in the reference
+(it replaces nothing there: the plain method stays the reference's
+trace.rs:80-120 mixture), so this restatement is pinned by mathematics, not
+by reference vectors:
+  * grad() against central finite differences of loss() (tests/test_guide_oracle.py);
+  * vmf_pdf() against numerical integration over the sphere (integrates to 1);
+  * vmf_sample() against vmf_pdf() by a chi-square histogram test.
+Only tests/ may import this module.
+"""
+import numpy as np
+
+IN, HID, LOBES, OUT, OUT_PAD, FREQS = 64, 64, 8, 40, 48, 9
+LOGK_MIN, LOGK_MAX = -3.0, 8.5
+
+# parameter vector layout (guide.h GD_W1 ... GD_B4)
+W1 = 0
+B1 = W1 + HID * IN
+W2 = B1 + HID
+B2 = W2 + HID * HID
+W3 = B2 + HID
+B3 = W3 + HID * HID
+W4 = B3 + HID
+B4 = W4 + OUT_PAD * HID
+NPARAM = B4 + OUT_PAD
+assert NPARAM == 15600
+
+
+def unpack(P):
+    P = np.asarray(P, np.float64)
+    return [(P[W1:B1].reshape(HID, IN), P[B1:W2]), (P[W2:B2].reshape(HID, HID), P[B2:W3]),
+            (P[W3:B3].reshape(HID, HID), P[B3:W4]), (P[W4:B4].reshape(OUT_PAD, HID), P[B4:NPARAM])]
+
+
+def encode(inp):
+    """inp [n, 7] = p xyz (scene box -> [-1, 1]), normal xyz, (lambda - 380) / 400 -> features [n, 64]"""
+    inp = np.asarray(inp, np.float64)
+    p, nrm, lam = inp[:, 0:3], inp[:, 3:6], inp[:, 6]
+    f = np.zeros((inp.shape[0], IN))
+    for k in range(FREQS):
+        a = p * (np.pi * 2.0 ** k)  # the device takes sin(2 pi * 2^(k-1) x) in revolutions: the same value
+        f[:, 6 * k:6 * k + 3] = np.sin(a)
+        f[:, 6 * k + 3:6 * k + 6] = np.cos(a)
+    f[:, 54:57] = p
+    f[:, 57:60] = nrm
+    f[:, 60] = lam
+    f[:, 61] = np.sin(2 * np.pi * lam)
+    f[:, 62] = np.cos(2 * np.pi * lam)
+    f[:, 63] = 1.0
+    return f
+
+
+def _h(x, fp16):
+    return x.astype(np.float16).astype(np.float64) if fp16 else x
+
+
+def forward(P, inp, fp16=False):
+    """raw outputs [n, 48] and the activations; fp16=True rounds weights and
+    inter-layer activations to fp16 like the device operands (accumulation
+    stays float64)."""
+    layers = unpack(P)
+    a = _h(encode(inp), fp16)
+    acts = [a]
+    for li, (W, b) in enumerate(layers):
+        z = a @ _h(W, fp16).T + b
+        if li < 3:
+            a = _h(np.maximum(z, 0.0), fp16)
+            acts.append(a)
+        else:
+            out = z
+    return out, acts
+
+
+def decode(o):
+    """o [n, >=40] -> mu [n, 8, 3], kappa [n, 8], w [n, 8], |m| [n, 8]"""
+    o = np.asarray(o, np.float64)
+    m = o[:, :OUT].reshape(-1, LOBES, 5)
+    raw_mu = m[:, :, 0:3]
+    ln = np.linalg.norm(raw_mu, axis=-1)
+    mu = np.where(ln[..., None] > 1e-12, raw_mu / np.maximum(ln, 1e-300)[..., None], np.array([0.0, 0.0, 1.0]))
+    kappa = np.exp(np.clip(m[:, :, 3], LOGK_MIN, LOGK_MAX))
+    lg = m[:, :, 4]
+    e = np.exp(lg - lg.max(axis=1, keepdims=True))
+    w = e / e.sum(axis=1, keepdims=True)
+    return mu, kappa, w, ln
+
+
+def vmf_norm(k):
+    return k / (2 * np.pi * -np.expm1(-2 * k))
+
+
+def vmf_pdf(mu, kappa, w, d):
+    """mixture density; mu [n, 8, 3], kappa/w [n, 8], d [n, 3] -> [n]"""
+    c = np.einsum("njk,nk->nj", mu, d)
+    return np.sum(w * vmf_norm(kappa) * np.exp(kappa * (c - 1.0)), axis=1)
+
+
+def vmf_sample(mu, kappa, w, u):
+    """the device's sampling procedure (guide.h gd_sample) for one query:
+    mu [8, 3], kappa/w [8], u [m, 3] uniforms -> directions [m, 3]"""
+    cw = np.cumsum(w)[:-1]
+    j = np.searchsorted(cw, u[:, 0], side="right")
+    k = kappa[j]
+    m = mu[j]
+    cz = 1.0 + np.log(u[:, 1] + (1.0 - u[:, 1]) * np.exp(-2.0 * k)) / k
+    cz = np.clip(cz, -1.0, 1.0)
+    s = np.sqrt(np.maximum(0.0, 1.0 - cz * cz))
+    phi = 2 * np.pi * u[:, 2]
+    vx, vy = s * np.cos(phi), s * np.sin(phi)
+    mx, my, mz = m[:, 0], m[:, 1], m[:, 2]
+    sg = np.copysign(1.0, mz)
+    a = -1.0 / (sg + mz)
+    bb = mx * my * a
+    t1 = np.stack([1.0 + sg * mx * mx * a, sg * bb, -sg * mx], -1)
+    t2 = np.stack([bb, sg + my * my * a, -my], -1)
+    d = vx[:, None] * t1 + vy[:, None] * t2 + cz[:, None] * m
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def loss(P, samples, fp16=False):
+    """mean over samples of -(w / mean w) log q(d); samples [n, 11] =
+    p xyz, normal xyz, lambda, d xyz, weight"""
+    s = np.asarray(samples, np.float64)
+    o, _ = forward(P, s[:, 0:7], fp16)
+    mu, kappa, w, _ = decode(o)
+    q = vmf_pdf(mu, kappa, w, s[:, 7:10])
+    wt = s[:, 10] / s[:, 10].mean()
+    return float(np.mean(-wt * np.log(np.maximum(q, 1e-30))))
+
+
+def output_grad(o, d, wt):
+    """d(-wt log q)/d(o) per sample, [n, 48] (guide.hip gd_loss_grad)"""
+    mu, kappa, w, ln = decode(o)
+    c = np.einsum("njk,nk->nj", mu, d)
+    f = vmf_norm(kappa) * np.exp(kappa * (c - 1.0))
+    q = np.maximum(np.sum(w * f, axis=1), 1e-30)
+    r = w * f / q[:, None]
+    g = np.zeros((o.shape[0], OUT_PAD))
+    gv = g[:, :OUT].reshape(-1, LOBES, 5)
+    gv[:, :, 4] = -wt[:, None] * (r - w)
+    s = np.asarray(o, np.float64)[:, :OUT].reshape(-1, LOBES, 5)[:, :, 3]
+    inside = (s > LOGK_MIN) & (s < LOGK_MAX)
+    with np.errstate(over="ignore"):
+        dlogf = 1.0 / kappa - 2.0 / np.expm1(2.0 * kappa) + (c - 1.0)
+    gv[:, :, 3] = np.where(inside, -wt[:, None] * r * kappa * dlogf, 0.0)
+    gmu = -(wt[:, None] * r * kappa)[..., None] * d[:, None, :]
+    mg = np.sum(mu * gmu, axis=-1, keepdims=True)
+    gm = (gmu - mu * mg) / np.maximum(ln, 1e-300)[..., None]
+    gv[:, :, 0:3] = np.where((ln > 1e-12)[..., None], gm, 0.0)
+    g[:, :OUT] = gv.reshape(-1, OUT)
+    return g
+
+
+def grad(P, samples, fp16=False):
+    """gradient of loss() w.r.t. the parameter vector (manual back-propagation)"""
+    s = np.asarray(samples, np.float64)
+    n = s.shape[0]
+    layers = unpack(P)
+    o, acts = forward(P, s[:, 0:7], fp16)
+    wt = s[:, 10] / s[:, 10].mean()
+    D = output_grad(o, s[:, 7:10], wt) / n
+    G = np.zeros(NPARAM)
+    offs = [(W1, B1), (W2, B2), (W3, B3), (W4, B4)]
+    for li in range(3, -1, -1):
+        W, _ = layers[li]
+        wo, bo = offs[li]
+        G[wo:wo + W.size] = (D.T @ acts[li]).ravel()
+        G[bo:bo + W.shape[0]] = D.sum(axis=0)
+        if li > 0:
+            D = (D @ _h(W, fp16)) * (acts[li] > 0)
+            if fp16:
+                D = _h(D * n, True) / n
+    return G
+
+
+def init_params(seed=1):
+    """random parameters of the same shape (not the device's initialiser)"""
+    rng = np.random.default_rng(seed)
+    P = np.zeros(NPARAM)
+    he = np.sqrt(2.0 / 64)
+    P[W1:B1] = rng.normal(0, he, HID * IN)
+    P[W2:B2] = rng.normal(0, he, HID * HID)
+    P[W3:B3] = rng.normal(0, he, HID * HID)
+    P[W4:W4 + OUT * HID] = rng.normal(0, 0.3, OUT * HID)
+    P[B1:W2] = rng.normal(0, 0.1, HID)
+    P[B2:W3] = rng.normal(0, 0.1, HID)
+    P[B3:W4] = rng.normal(0, 0.1, HID)
+    P[B4:B4 + OUT] = rng.normal(0, 0.5, OUT)
+    return P
+
+
+def random_samples(n, seed=2):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-1, 1, (n, 3))
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    lam = rng.integers(0, 400, n) / 400.0
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = rng.exponential(1.0, n) + 1e-3
+    return np.concatenate([p, nrm, lam[:, None], d, w[:, None]], axis=1)

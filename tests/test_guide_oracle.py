@@ -1,0 +1,141 @@
+"""Pins of the guiding-network oracle (oracle/guide_ref.py) by mathematics:
+finite differences for the back-propagated gradient, the closed-form
+normalisation of the vMF density, and the distribution of the sampler."""
+import numpy as np
+import pytest
+from scipy import integrate, stats
+
+from oracle import guide_ref as G
+
+
+def test_parameter_layout_matches_the_library_header():
+    text = open(__file__.replace("tests/test_guide_oracle.py", "include/clovers_hip.h")).read()
+    assert "#define CLV_GUIDE_NPARAM %du" % G.NPARAM in text
+    assert "#define CLV_GUIDE_OUT %du" % G.OUT in text
+
+
+def test_output_gradient_matches_finite_differences():
+    rng = np.random.default_rng(5)
+    o = rng.normal(0, 1.0, (6, G.OUT_PAD))
+    o[:, 3:G.OUT:5] = rng.uniform(-2.5, 8.0, (6, G.LOBES))  # log kappa inside the clamp
+    d = rng.normal(size=(6, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    wt = rng.uniform(0.5, 2.0, 6)
+
+    def f(oo):
+        mu, k, w, _ = G.decode(oo)
+        return np.sum(-wt * np.log(G.vmf_pdf(mu, k, w, d)))
+    g = G.output_grad(o, d, wt)
+    h = 1e-6
+    for i in range(6):
+        for j in range(G.OUT):
+            e = np.zeros_like(o)
+            e[i, j] = h
+            fd = (f(o + e) - f(o - e)) / (2 * h)
+            assert abs(fd - g[i, j]) <= 1e-5 * max(1.0, abs(fd)), (i, j, fd, g[i, j])
+    assert np.all(g[:, G.OUT:] == 0)
+
+
+def test_clamped_log_kappa_has_no_gradient():
+    o = np.zeros((1, G.OUT_PAD))
+    o[0, 3] = G.LOGK_MAX + 1.0
+    o[0, 8] = G.LOGK_MIN - 1.0
+    g = G.output_grad(o, np.array([[0.0, 0.0, 1.0]]), np.ones(1))
+    assert g[0, 3] == 0 and g[0, 8] == 0
+
+
+def test_parameter_gradient_matches_finite_differences():
+    P = G.init_params(3)
+    s = G.random_samples(24, seed=4)
+    g = G.grad(P, s)
+    rng = np.random.default_rng(6)
+    idx = np.concatenate([rng.choice(np.arange(G.W1, G.B1), 40), np.arange(G.B1, G.B1 + 5),
+                          rng.choice(np.arange(G.W2, G.B2), 30), rng.choice(np.arange(G.W3, G.B3), 30),
+                          rng.choice(np.arange(G.W4, G.W4 + G.OUT * G.HID), 60), np.arange(G.B4, G.B4 + G.OUT)])
+    h = 1e-6
+    worst = 0.0
+    for i in idx:
+        e = np.zeros_like(P)
+        e[i] = h
+        fd = (G.loss(P + e, s) - G.loss(P - e, s)) / (2 * h)
+        worst = max(worst, abs(fd - g[i]) / max(1e-3, abs(fd)))
+    assert worst < 1e-4, worst
+    # the padding rows 40..47 of the output layer get no gradient
+    assert np.all(g[G.W4 + G.OUT * G.HID:G.B4] == 0) and np.all(g[G.B4 + G.OUT:] == 0)
+
+
+@pytest.mark.parametrize("kappa", [0.05, 1.0, 30.0, 4915.0])
+def test_vmf_lobe_integrates_to_one(kappa):
+    # in the lobe's frame the density depends on t = mu.w only: 2 pi * int_{-1}^{1} c e^{k(t-1)} dt
+    c = G.vmf_norm(np.float64(kappa))
+    f = lambda t: 2 * np.pi * c * np.exp(kappa * (t - 1.0))
+    cut = max(-1.0, 1.0 - 60.0 / kappa)
+    val = integrate.quad(f, cut, 1.0, limit=200)[0] + (integrate.quad(f, -1.0, cut, limit=200)[0] if cut > -1 else 0)
+    assert abs(val - 1.0) < 1e-8
+
+
+def test_mixture_integrates_to_one_on_the_sphere():
+    # Fibonacci lattice quadrature of a mixture with moderate kappa
+    N = 400000
+    i = np.arange(N) + 0.5
+    z = 1 - 2 * i / N
+    phi = np.pi * (1 + 5 ** 0.5) * i
+    r = np.sqrt(1 - z * z)
+    dirs = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+    rng = np.random.default_rng(1)
+    mu = rng.normal(size=(G.LOBES, 3))
+    mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+    kappa = np.array([0.05, 0.5, 2.0, 5.0, 20.0, 50.0, 100.0, 200.0])
+    w = rng.dirichlet(np.ones(G.LOBES))
+    q = G.vmf_pdf(np.broadcast_to(mu, (N, G.LOBES, 3)), np.broadcast_to(kappa, (N, G.LOBES)),
+                  np.broadcast_to(w, (N, G.LOBES)), dirs)
+    assert abs(q.mean() * 4 * np.pi - 1.0) < 2e-3
+
+
+def test_sampler_follows_the_density():
+    rng = np.random.default_rng(2)
+    mu = rng.normal(size=(G.LOBES, 3))
+    mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+    mu[0] = [0.0, 0.0, -1.0]  # the basis' special case
+    kappa = np.array([0.1, 3.0, 40.0, 900.0, 1.0, 10.0, 4915.0, 0.05])
+    w = rng.dirichlet(np.ones(G.LOBES))
+    n = 400000
+    u = rng.random((n, 3))
+    d = G.vmf_sample(mu, kappa, w, u)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0)
+    # lobe choice frequencies follow the weights
+    j = np.searchsorted(np.cumsum(w)[:-1], u[:, 0], side="right")
+    counts = np.bincount(j, minlength=G.LOBES)
+    assert stats.chisquare(counts, w * n).pvalue > 1e-3
+    # within each lobe, t = mu.w has the vMF cdf (e^{k(t-1)} - e^{-2k}) / (1 - e^{-2k}),
+    # and the azimuth about mu is uniform
+    for lobe in range(G.LOBES):
+        sel = j == lobe
+        t = d[sel] @ mu[lobe]
+        k = kappa[lobe]
+        cdf = lambda x: (np.exp(k * (np.clip(x, -1, 1) - 1)) - np.exp(-2 * k)) / -np.expm1(-2 * k)
+        assert stats.kstest(t, cdf).pvalue > 1e-4, lobe
+        a = np.cross(mu[lobe], [1.0, 0.0, 0.0] if abs(mu[lobe][0]) < 0.9 else [0.0, 1.0, 0.0])
+        a /= np.linalg.norm(a)
+        b = np.cross(mu[lobe], a)
+        az = np.arctan2(d[sel] @ b, d[sel] @ a)
+        if k < 1000:  # at the largest kappa the azimuth is not resolved in float64 of 1 - t
+            assert stats.kstest(az, stats.uniform(-np.pi, 2 * np.pi).cdf).pvalue > 1e-4, lobe
+
+
+def test_decode_gives_a_valid_mixture():
+    rng = np.random.default_rng(3)
+    o = rng.normal(0, 5, (50, G.OUT_PAD))
+    mu, k, w, _ = G.decode(o)
+    assert np.allclose(np.linalg.norm(mu, axis=-1), 1.0)
+    assert np.all(k >= np.exp(G.LOGK_MIN) * 0.999) and np.all(k <= np.exp(G.LOGK_MAX) * 1.001)
+    assert np.allclose(w.sum(1), 1.0)
+
+
+def test_encoding_layout():
+    inp = np.array([[0.25, -0.5, 0.75, 0.0, 1.0, 0.0, 0.3]])
+    f = G.encode(inp)[0]
+    assert np.isclose(f[0], np.sin(np.pi * 0.25)) and np.isclose(f[4], np.cos(np.pi * -0.5))
+    assert np.isclose(f[6 + 2], np.sin(2 * np.pi * 0.75))
+    assert np.allclose(f[54:57], inp[0, :3]) and np.allclose(f[57:60], inp[0, 3:6])
+    assert f[60] == 0.3 and f[63] == 1.0
