@@ -29,7 +29,12 @@ WORKLOADS = {
     "cornell": dict(scene="cornell", width=256, height=256, samples=16, max_depth=8, views=1, count_spp=16),
     # 1,003,520-triangle noise-displaced icosphere, 4 camera views in one batch
     "mesh": dict(scene="mesh", width=1920, height=1080, samples=256, max_depth=12, views=4, count_spp=8),
+    # dispersive prism caustics: equal-RMSE time of the guided and the plain method
+    "prism": dict(scene="prism", width=1024, height=1024, samples=512, max_depth=24, views=1, count_spp=0),
 }
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+# network FLOPs per evaluation: 2 x (64x64 x 3 hidden + 40x64 output) multiply-adds
+GUIDE_FLOP = 2 * (64 * 64 * 3 + 40 * 64)
 HBM_PEAK_GBS = 8000.0
 TRACE_KERNELS = {0: "k_trace_simple", 1: "k_trace", 2: "k_trace_simple (ww)", 3: "k_trace_lds"}
 RAY_STREAM_BYTES = 48  # per traced ray: queue index in/out (8), origin+direction (32), hit t+record (8)
@@ -87,6 +92,95 @@ def cpu_baseline(scene, w, seconds):
                       % (rows, w["width"], w["height"], w["samples"], w["max_depth"], samples, dt, kind_note, threads)}
 
 
+def time_to_rmse(curve, target):
+    """device ms at which an RMSE-vs-time curve [(ms, rmse)] first reaches
+    `target`: log-log interpolation between points, or an extrapolation
+    from the last point with RMSE ~ t^-1/2 if it never does"""
+    import math
+    prev = None
+    for t, e in curve:
+        if e <= target:
+            if prev is None:
+                return t * (e / target) ** 2, False
+            (t0, e0), (t1, e1) = prev, (t, e)
+            if e0 == e1:
+                return t1, False
+            a = (math.log(target) - math.log(e0)) / (math.log(e1) - math.log(e0))
+            return math.exp(math.log(t0) + a * (math.log(t1) - math.log(t0))), False
+        prev = (t, e)
+    t, e = curve[-1]
+    return t * (e / target) ** 2, True
+
+
+def run_prism(args):
+    """Time-to-RMSE of guided vs plain rendering on the dispersive-prism scene:
+    both methods render independent images at growing sample counts (each
+    guided render trains its network online from scratch), the RMSE of
+    luminance against a float64-accumulated plain reference is taken per
+    image, and the device time the guided method needs to reach the plain
+    method's RMSE at its largest sample count is read off the guided curve."""
+    import numpy as np
+
+    import clovers_amd
+    from clovers_amd import abi, synth
+    w = dict(WORKLOADS["prism"])
+    W, H, D = w["width"], w["height"], w["max_depth"]
+    spps = [s for s in (16, 32, 64, 128, 256, 512) if s <= (args.spp or w["samples"])]
+    r = clovers_amd.Renderer(0)
+    sid = r.upload(synth.prism())
+    if args.guide_alpha:
+        r.set_tuning(guide_alpha=args.guide_alpha)
+    if args.guide_rounds:
+        r.set_tuning(guide_rounds=args.guide_rounds)
+    tuning = r.autotune([(sid, abi.render_opts(W, H, spps[-1], D, seed=1))])
+    log("autotune: %s" % tuning)
+    t0 = time.perf_counter()
+    ref, _ = r.render(sid, abi.render_opts(W, H, args.ref_spp, D, seed=99, fp64_accum=1))
+    log("reference: %d spp in %.1f s" % (args.ref_spp, time.perf_counter() - t0))
+    refY = ref[..., 1].astype(np.float64)
+    curves = {}
+    for name, g in (("plain", abi.GUIDE_OFF), ("guided", abi.GUIDE_ON)):
+        pts = []
+        for spp in spps:
+            mse, ms = 0.0, 0.0
+            seeds = (7, 8)
+            for sd in seeds:
+                img, _ = r.render(sid, abi.render_opts(W, H, spp, D, seed=sd, guiding=g))
+                ms += r.counters()["render_ms"] / len(seeds)
+                mse += np.mean((img[..., 1].astype(np.float64) - refY) ** 2) / len(seeds)
+            pts.append((round(ms, 3), float(np.sqrt(mse))))
+            log("%-6s %4d spp: %9.1f ms  RMSE %.5f" % (name, spp, ms, pts[-1][1]))
+        curves[name] = pts
+    t_plain, target = curves["plain"][-1]
+    t_guided, extrapolated = time_to_rmse(curves["guided"], target)
+    ratio = t_plain / t_guided
+    # the network's MFMA roofline: evaluations x FLOPs over the inference kernels' device time
+    r.render(sid, abi.render_opts(W, H, spps[-1], D, seed=7, guiding=abi.GUIDE_ON, flags=abi.FLAG_TIME_KERNELS))
+    cg = r.counters()
+    achieved = cg["guide_queries"] * GUIDE_FLOP / (cg["guide_infer_ms"] * 1e-3) / 1e12 if cg["guide_infer_ms"] else 0.0
+    r.close()
+    line = {
+        "metric": "time-to-RMSE vs oracle, guided vs plain", "value": round(ratio, 3),
+        "unit": "x (plain device time / guided device time at equal RMSE)", "n_gpus": 1, "steps": 2 * len(spps),
+        "warmup": 0, "ms_per_step": round(t_guided, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (network fp16 on MFMA, fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": "prism %dx%d depth%d" % (W, H, D), "reference_spp": args.ref_spp,
+                   "reference": "plain method, float64 accumulation", "target_rmse": target,
+                   "plain_ms_at_target": t_plain, "guided_ms_at_target": round(t_guided, 3),
+                   "guided_extrapolated": extrapolated, "curves_ms_rmse": curves, "seeds_per_point": 2,
+                   "tuning": dict(tuning, trace_kernel=abi.TRACE_NAMES[tuning["trace_kernel"]])},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_FP16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / MFMA_FP16_PEAK_TFLOPS, 5), "traffic": None,
+                     "kernel": "k_guide_infer (network evaluation, fp16 MFMA)",
+                     "evaluations": cg["guide_queries"], "flop_per_evaluation": GUIDE_FLOP,
+                     "infer_ms": round(cg["guide_infer_ms"], 3), "train_ms": round(cg["guide_train_ms"], 3),
+                     "train_records": cg["guide_records"], "train_rounds": cg["guide_rounds"],
+                     "render_ms": round(cg["render_ms"], 3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +190,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--ref-spp", type=int, default=65536, help="prism workload: samples of the reference image")
+    ap.add_argument("--guide-alpha", type=float, default=0.0, help="prism workload: override the guide's share")
+    ap.add_argument("--guide-rounds", type=int, default=0, help="prism workload: override the training rounds")
     ap.add_argument("--spp", type=int, default=0,
                     help="override the workload's samples per pixel (profiling runs: per-launch figures do not "
                          "depend on it; the JSON line then names the reduced workload)")
@@ -107,6 +204,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (the work would shard by "
               "sample range with one all-reduce of the pixel sums at the end)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "prism":
+        run_prism(args)
+        return
 
     import numpy as np
 

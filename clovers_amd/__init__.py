@@ -246,7 +246,7 @@ class Renderer:
     def counters(self):
         c = abi.Counters()
         self._check(self.lib.clv_counters_get(self.h, C.byref(c)))
-        return {k: getattr(c, k) for k, _ in abi.Counters._fields_}
+        return {k: getattr(c, k) for k, _ in abi.Counters._fields_ if k != "_pad"}
 
 
 def draw(scene, width, height, samples, max_depth, seed=1, device=0, **kw):

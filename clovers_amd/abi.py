@@ -86,7 +86,7 @@ class Tuning(C.Structure):
     _fields_ = [("target_paths", C.c_uint32), ("trace_kernel", C.c_uint32), ("lds_bytes", C.c_uint32),
                 ("shade_grid_cap", C.c_uint32), ("guide_train_spp", C.c_uint32), ("guide_alpha", C.c_float),
                 ("guide_lr", C.c_float), ("guide_records", C.c_uint32), ("guide_wclip", C.c_float),
-                ("guide_rounds", C.c_uint32), ("_pad", C.c_uint32 * 2)]
+                ("guide_rounds", C.c_uint32), ("use_graph", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 class Counters(C.Structure):
@@ -94,7 +94,9 @@ class Counters(C.Structure):
                 ("prim_tests", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),
                 ("build_ms", C.c_double), ("extend_launches", C.c_uint32), ("passes", C.c_uint32),
                 ("bvh_nodes", C.c_uint32), ("bvh_width", C.c_uint32), ("node_bytes", C.c_uint32),
-                ("prim_bytes", C.c_uint32), ("n_prims", C.c_uint32), ("bvh_depth", C.c_uint32)]
+                ("prim_bytes", C.c_uint32), ("n_prims", C.c_uint32), ("bvh_depth", C.c_uint32),
+                ("guide_queries", C.c_uint64), ("guide_records", C.c_uint64), ("guide_infer_ms", C.c_double),
+                ("guide_train_ms", C.c_double), ("guide_rounds", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):

@@ -81,7 +81,7 @@ struct GuideState {
   bool init = false, trained = false;
   /* training schedule (clv_tuning) */
   uint32_t train_spp = 4;
-  float alpha = 0.9f;
+  float alpha = 0.5f;
   float lr = 0.005f;
   uint32_t records = 262144;
   float wclip = 1e6f;
@@ -130,6 +130,13 @@ struct clv_ctx {
   uint32_t target_paths = 1u << 22;
   uint32_t grid_cap = 2048;
   GuideState guide;
+  /* the last render's launch sequence as an executable graph, replayed when
+   * an identical render (same signature) comes again */
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  std::string gsig;
+  uint32_t g_stamps = 0, g_t_after = 0, g_rounds = 0;
+  bool g_trained_after = false;
 };
 
 namespace {
@@ -491,8 +498,9 @@ clv_status guide_reset(clv_ctx *c, uint64_t seed) {
 /* one training round on the records of pass `stamp` (n path slots) */
 void guide_train_round(clv_ctx *c, uint32_t paths, uint32_t stamp, float *grad_out, float wclip) {
   GuideState &g = c->guide;
-  launch_guide_collect(c->stream, g.rec, paths * GD_SLOTS, stamp, g.bcount, g.bweight, g.totals, g.list);
   const uint32_t cap = grad_out ? paths * GD_SLOTS : g.records;
+  launch_guide_collect(c->stream, g.rec, paths * GD_SLOTS, stamp, g.bcount, g.bweight, g.totals, g.list, cap,
+                       (unsigned long long *)c->stats + 4);
   const uint32_t step = grad_out ? cap : GUIDE_STEP;
   for (uint32_t b = 0; b < cap; b += step) {
     if (!grad_out) g.t++;
@@ -581,9 +589,6 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   ShadeQueues sq;
   for (uint32_t q = 0; q < MC_COUNT; q++) sq.q[q] = c->shadeq[q];
   sq.counts = c->counts + CNT_SHADE;
-  HIPCHK(c, hipMemsetAsync(c->stats, 0, 4 * sizeof(uint64_t), c->stream));
-  launch_zero_accum(c->stream, rp);
-  HIPCHK(c, hipEventRecord(c->events[0], c->stream));
   size_t ev = 2;
   PathBuf pb = c->pb;
   pb.gcnt = any_guide ? gs.gcnt : nullptr;
@@ -595,7 +600,26 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     ga.Wh = gs.Wh;
     ga.P = gs.P;
     ga.on = 1u;
+    ga.stats = (unsigned long long *)c->stats;
   }
+  /* timing events of the guide's launches: [start, end) pairs */
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> gev_infer, gev_train;
+  auto guide_event = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>> &v) -> hipEvent_t {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess) return nullptr;
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return nullptr; }
+    v.push_back({a, b});
+    (void)hipEventRecord(a, c->stream);
+    return b;
+  };
+  uint32_t rounds_run = 0;
+  bool any_sq = false;
+  for (uint32_t k = 0; k < n; k++) any_sq = any_sq || jobs[k].xyz_sq_out;
+  /* every launch of the render; no host synchronisation inside, so the
+   * sequence can be captured as a graph */
+  auto enqueue = [&]() -> clv_status {
+  launch_zero_u64(c->stream, c->stats, 8);
+  launch_zero_accum(c->stream, rp);
   for (uint32_t p = 0; p < passes; p++) {
     GenParams gp;
     memset(&gp, 0, sizeof(gp));
@@ -630,8 +654,11 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
         launch_extend(c->stream, ds, pb, ta, geom.blocks, counting, c->trace_variant);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
       ev += timing ? 2 : 0;
-      if (any_guide && ga.alpha > 0.0f)
+      if (any_guide && ga.alpha > 0.0f) {
+        hipEvent_t end = timing ? guide_event(gev_infer) : nullptr;
         launch_guide_infer(c->stream, ds, pb, ga, sq.q[MC_LAMBERTIAN], sq.counts + MC_LAMBERTIAN, gp.paths);
+        if (end) (void)hipEventRecord(end, c->stream);
+      }
       launch_shade(c->stream, ds, pb, sq, c->queue[out], c->counts, out_slot, gp.paths, c->grid_cap, ga);
     }
     rp.items = gp.items;
@@ -639,13 +666,82 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     rp.threads = pl.res[p];
     launch_resolve(c->stream, pb, rp, c->cie);
     if (any_guide && ga.record) {
+      hipEvent_t end = timing ? guide_event(gev_train) : nullptr;
       guide_train_round(c, gp.paths, ga.stamp, nullptr, gs.wclip);
+      if (end) (void)hipEventRecord(end, c->stream);
       gs.trained = true;
+      rounds_run++;
     }
   }
-  bool any_sq = false;
-  for (uint32_t k = 0; k < n; k++) any_sq = any_sq || jobs[k].xyz_sq_out;
   launch_finalize(c->stream, rp, c->out, any_sq ? c->out_sq : nullptr);
+  return CLV_OK;
+  };
+  const bool keep = (flags & CLV_FLAG_GUIDE_KEEP) != 0;
+  const bool use_graph = c->use_graph && !timing && !(any_guide && keep);
+  std::string sig;
+  if (use_graph) {
+    /* everything a launch argument depends on; pass ids of guided renders
+     * are left out: a replay reuses the captured ones, which only the same
+     * (deterministic) render can have written */
+    auto add = [&sig](const void *p, size_t b) { sig.append((const char *)p, b); };
+    add(dj.data(), dj.size() * sizeof(DevJob));
+    add(pl.items.data(), pl.items.size() * sizeof(DevItem));
+    add(pl.first.data(), pl.first.size() * 4);
+    add(pl.paths.data(), pl.paths.size() * 4);
+    add(pl.res.data(), pl.res.size() * 4);
+    add(pl.depth.data(), pl.depth.size() * 4);
+    add(pl.train.data(), pl.train.size());
+    const uint32_t sc[8] = {flags, any_sq, any_guide, frozen, (uint32_t)c->trace_variant, c->grid_cap, ds.has_xforms,
+                            gs.train_spp};
+    add(sc, sizeof(sc));
+    const float gf[4] = {gs.alpha, gs.lr, gs.wclip, (float)gs.records};
+    add(gf, sizeof(gf));
+    add(&geom, sizeof(geom));
+    add(&pb, sizeof(pb));
+    const void *ptrs[] = {c->queue[0], c->queue[1], c->shadeq[0], c->shadeq[1], c->shadeq[2], c->shadeq[3],
+                          c->shadeq[4], c->acc32, c->acc64, c->accsq, c->out, c->out_sq, c->counts, c->stats,
+                          c->spill, c->cie, c->jobs, c->items, gs.lobes, gs.rec, gs.gcnt, gs.list, gs.bcount,
+                          gs.bweight, gs.totals, gs.P, gs.M, gs.V, gs.Wh, gs.partial};
+    add(ptrs, sizeof(ptrs));
+    for (auto &a : c->arena) add(&a.p, sizeof(a.p));
+  }
+  HIPCHK(c, hipEventRecord(c->events[0], c->stream));
+  if (use_graph && c->gexec && c->gsig == sig) {
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    gs.stamp += c->g_stamps;
+    gs.t = c->g_t_after;
+    gs.trained = c->g_trained_after;
+    rounds_run = c->g_rounds;
+  } else {
+    bool capturing = false;
+    const uint32_t stamp0 = gs.stamp;
+    if (use_graph) {
+      if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+      c->gsig.clear();
+      capturing = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    }
+    clv_status st = enqueue();
+    if (capturing) {
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (st == CLV_OK && e == hipSuccess) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (st != CLV_OK) return st;
+      if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, CLV_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+      }
+      c->gsig = sig;
+      c->g_stamps = gs.stamp - stamp0;
+      c->g_t_after = gs.t;
+      c->g_trained_after = gs.trained;
+      c->g_rounds = rounds_run;
+      HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    } else if (st != CLV_OK) {
+      return st;
+    }
+  }
   HIPCHK(c, hipEventRecord(c->events[1], c->stream));
   HIPCHK(c, hipGetLastError());
   for (uint32_t k = 0; k < n; k++) {
@@ -655,12 +751,26 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
       HIPCHK(c, hipMemcpyAsync(jobs[k].xyz_sq_out, c->out_sq + off, sizeof(float) * cnt, hipMemcpyDeviceToHost,
                                c->stream));
   }
-  uint64_t stats[4];
+  uint64_t stats[8];
   HIPCHK(c, hipMemcpyAsync(stats, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0.0f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->events[0], c->events[1]));
   clv_counters &cn = c->counters;
+  cn.guide_queries = any_guide ? stats[3] : 0;
+  cn.guide_rounds = rounds_run;
+  cn.guide_records = any_guide ? stats[4] : 0;
+  cn.guide_infer_ms = cn.guide_train_ms = 0.0;
+  for (auto *v : {&gev_infer, &gev_train}) {
+    double tot = 0.0;
+    for (auto &e : *v) {
+      float m = 0.0f;
+      if (hipEventElapsedTime(&m, e.first, e.second) == hipSuccess) tot += m;
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    (v == &gev_infer ? cn.guide_infer_ms : cn.guide_train_ms) = tot;
+  }
   cn.samples = samples;
   cn.rays = stats[2];
   cn.node_visits = counting ? stats[0] : 0;
@@ -731,7 +841,7 @@ clv_status clv_create(int hip_device, clv_ctx **out) {
   if (const char *e = getenv("CLV_TRACE_GRID")) c->full_grid = strcmp(e, "full") == 0;
   if (c->trace_blocks < 1) c->trace_blocks = 1;
   if (const char *e = getenv("CLV_LDS_BYTES")) c->lds_block_bytes = (size_t)strtoul(e, nullptr, 10);
-  if (dalloc(&c->counts, CNT_TOTAL) != hipSuccess || dalloc(&c->stats, 4) != hipSuccess ||
+  if (dalloc(&c->counts, CNT_TOTAL) != hipSuccess || dalloc(&c->stats, 8) != hipSuccess ||
       dalloc(&c->cie, CLV_SPECTRUM) != hipSuccess) {
     clv_destroy(c);
     return CLV_ENOMEM;
@@ -766,6 +876,7 @@ void clv_destroy(clv_ctx *c) {
   dfree(c->spill);
   dfree(c->cie);
   for (auto e : c->events) (void)hipEventDestroy(e);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -930,6 +1041,7 @@ clv_status clv_tuning_get(const clv_ctx *c, clv_tuning *out) {
   out->guide_records = c->guide.records;
   out->guide_wclip = c->guide.wclip;
   out->guide_rounds = c->guide.rounds;
+  out->use_graph = c->use_graph ? 1u : 0u;
   return CLV_OK;
 }
 
@@ -955,6 +1067,7 @@ clv_status clv_tuning_set(clv_ctx *c, const clv_tuning *t) {
   c->guide.records = t->guide_records;
   c->guide.wclip = t->guide_wclip;
   c->guide.rounds = t->guide_rounds;
+  c->use_graph = t->use_graph != 0;
   return CLV_OK;
 }
 
@@ -1047,7 +1160,7 @@ clv_status clv_guide_grad(clv_ctx *c, const float *samples, uint32_t n, float *g
   HIPCHK(c, dalloc(&dgrad, GD_NPARAM));
   hipError_t e = hipMemcpyAsync(g.rec, recs.data(), n * sizeof(GuideRec), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
-    launch_guide_collect(c->stream, g.rec, n, stamp, g.bcount, g.bweight, g.totals, g.list);
+    launch_guide_collect(c->stream, g.rec, n, stamp, g.bcount, g.bweight, g.totals, g.list, n, nullptr);
     launch_guide_step(c->stream, g.rec, g.list, g.totals, n, 0, n, FLT_MAX, g.P, g.M, g.V, g.Wh, g.partial, g.lr, 1,
                       dgrad);
     e = hipGetLastError();

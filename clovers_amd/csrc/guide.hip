@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_guide_count(const GuideRec *rec, uint32
 /* one block: exclusive scan of the block counts; totals[0] = valid records,
  * totals[1] = their mean weight (as float bits) */
 __global__ __launch_bounds__(1024) void k_guide_scan(uint32_t *bcount, const float *bweight, uint32_t nb,
-                                                     uint32_t *totals) {
+                                                     uint32_t *totals, uint32_t cap, unsigned long long *used) {
   __shared__ uint32_t part[1024];
   __shared__ double wpart[1024];
   const uint32_t per = (nb + 1023u) / 1024u;
@@ -91,6 +91,7 @@ __global__ __launch_bounds__(1024) void k_guide_scan(uint32_t *bcount, const flo
     totals[0] = acc;
     float mean = acc ? (float)(wacc / (double)acc) : 0.0f;
     totals[1] = __float_as_uint(mean);
+    if (used) *used += (unsigned long long)min(acc, cap);
   }
   __syncthreads();
   uint32_t run = part[threadIdx.x];
@@ -428,10 +429,10 @@ __global__ __launch_bounds__(64) void k_guide_eval(const float *in, uint32_t n, 
 
 /* ---------------------------------------------------------- launchers --- */
 void launch_guide_collect(hipStream_t st, const GuideRec *rec, uint32_t n, uint32_t stamp, uint32_t *bcount,
-                          float *bweight, uint32_t *totals, uint32_t *list) {
+                          float *bweight, uint32_t *totals, uint32_t *list, uint32_t cap, unsigned long long *used) {
   const uint32_t nb = (n + 255u) / 256u;
   hipLaunchKernelGGL(k_guide_count, dim3(nb), dim3(256), 0, st, rec, n, stamp, bcount, bweight);
-  hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(1024), 0, st, bcount, bweight, nb, totals);
+  hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(1024), 0, st, bcount, bweight, nb, totals, cap, used);
   hipLaunchKernelGGL(k_guide_scatter, dim3(nb), dim3(256), 0, st, rec, n, stamp, bcount, list);
 }
 

@@ -96,6 +96,7 @@ struct GuideArgs {
   uint32_t stamp;          /* pass id written into records */
   uint32_t record;         /* write training records */
   uint32_t on;             /* the batch has guided jobs: lobes/rec/gcnt are valid */
+  unsigned long long *stats; /* stats[3] counts network evaluations */
 };
 
 struct ShadeQueues {
@@ -138,9 +139,11 @@ struct ResolveParams {
   double *acc_sq;         /* 3 per pixel */
 };
 
-/* stats: 4 x u64 = node visits, primitive tests, rays traced, spare */
+/* stats: 8 x u64 = node visits, primitive tests, rays traced, network evaluations, training records used */
 void launch_reset_counts(hipStream_t st, uint32_t *counts, uint32_t in_slot, uint32_t out_slot, uint64_t *stats);
 void launch_set_count(hipStream_t st, uint32_t *counts, uint32_t slot, uint32_t value);
+/* zero n 64-bit words with a kernel (a memset node misbehaves in captured graphs) */
+void launch_zero_u64(hipStream_t st, uint64_t *p, uint32_t n);
 void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp, uint32_t *qout);
 /* persistent traversal: grid = resident blocks of the device; spill buffer of
  * trace_spill_words(blocks) words */
@@ -172,7 +175,7 @@ void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, c
                         const uint32_t *queue, const uint32_t *count, uint32_t max_paths);
 /* guide.hip: record compaction, training steps, network probes */
 void launch_guide_collect(hipStream_t st, const GuideRec *rec, uint32_t n, uint32_t stamp, uint32_t *bcount,
-                          float *bweight, uint32_t *totals, uint32_t *list);
+                          float *bweight, uint32_t *totals, uint32_t *list, uint32_t cap, unsigned long long *used);
 void launch_guide_step(hipStream_t st, const GuideRec *rec, const uint32_t *list, const uint32_t *totals,
                        uint32_t cap, uint32_t step_base, uint32_t step_len, float wclip, float *P, float *m,
                        float *v, _Float16 *Wh, float *partial, float lr, uint32_t t, float *grad_out);

@@ -370,6 +370,10 @@ __global__ void k_reset_counts(uint32_t *counts, uint32_t in_slot, uint32_t out_
   if (i == 1) counts[CNT_FETCH] = 0u;
 }
 
+__global__ void k_zero_u64(uint64_t *p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
 __global__ void k_set_count(uint32_t *counts, uint32_t slot, uint32_t value) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counts[slot] = value;
 }
@@ -1174,6 +1178,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, Gu
   }
   const uint32_t n = *count;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ga.stats) atomicAdd(ga.stats + 3, (unsigned long long)n);
   for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
     const uint32_t qi = base + threadIdx.x;
     uint32_t path = 0xFFFFFFFFu;
@@ -1331,6 +1336,10 @@ static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
 void launch_reset_counts(hipStream_t st, uint32_t *counts, uint32_t in_slot, uint32_t out_slot, uint64_t *stats) {
   hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, counts, in_slot, out_slot, (unsigned long long *)stats);
 }
+void launch_zero_u64(hipStream_t st, uint64_t *p, uint32_t n) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
+}
+
 void launch_set_count(hipStream_t st, uint32_t *counts, uint32_t slot, uint32_t value) {
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, st, counts, slot, value);
 }

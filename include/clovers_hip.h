@@ -180,6 +180,12 @@ typedef struct {
   uint32_t prim_bytes;       /* bytes of primitive records / number of primitives */
   uint32_t n_prims;          /* primitives after flattening (a Boxy is 6) */
   uint32_t bvh_depth;        /* depth of the wide BVH */
+  uint64_t guide_queries;    /* guided renders: network evaluations (one per guided diffuse vertex) */
+  uint64_t guide_records;    /* guided renders: training records used (summed over steps) */
+  double guide_infer_ms;     /* device time of the network evaluations (CLV_FLAG_TIME_KERNELS) */
+  double guide_train_ms;     /* device time of the training rounds (CLV_FLAG_TIME_KERNELS) */
+  uint32_t guide_rounds;     /* training rounds run */
+  uint32_t _pad;
 } clv_counters;
 
 typedef struct clv_ctx clv_ctx;
@@ -260,13 +266,15 @@ typedef struct {
   uint32_t shade_grid_cap;  /* largest grid of the shading kernel, in 256-thread blocks */
   uint32_t guide_train_spp; /* guided jobs: samples per pixel between training rounds (default 4) */
   float guide_alpha;        /* at a diffuse vertex, share of the cosine half of the mixture drawn from the
-                               learned lobes (default 0.9) */
+                               learned lobes (default 0.5) */
   float guide_lr;           /* Adam learning rate (default 0.005) */
   uint32_t guide_records;   /* training records used per round (default 262144, in steps of 16384) */
   float guide_wclip;        /* training weight F/p capped at this multiple of its mean (default 1e6) */
   uint32_t guide_rounds;    /* training rounds per guided render (default 16); later samples use the
                                trained network in large passes without training */
-  uint32_t _pad[2];
+  uint32_t use_graph;       /* 1 (default): capture a render's launches as a hipGraph and replay it for an
+                               identical render; never for CLV_FLAG_TIME_KERNELS or guided CLV_FLAG_GUIDE_KEEP */
+  uint32_t _pad;
 } clv_tuning;
 clv_status clv_tuning_get(const clv_ctx *ctx, clv_tuning *out);
 clv_status clv_tuning_set(clv_ctx *ctx, const clv_tuning *t);

@@ -129,3 +129,25 @@ def test_autotune_keeps_a_valid_tuning():
         assert np.isfinite(img).all()
         with pytest.raises(clovers_amd.ClvError):
             r.set_tuning(target_paths=5)
+
+
+def test_graph_replays_match_eager_renders():
+    """A render's launches are captured as a graph and replayed for an
+    identical render; every replay, with eager work in between, must give the
+    eager bytes (plain batches and a guided render with online training)."""
+    with clovers_amd.Renderer(0) as r:
+        a = r.upload(synth.cornell())
+        p = r.upload(synth.prism())
+        m = r.upload(tiny_mesh())
+        jobs = [(a, abi.render_opts(96, 64, 6, 8, seed=3)), (m, abi.render_opts(80, 60, 5, 12, seed=4))]
+        gjobs = [(p, abi.render_opts(64, 64, 24, 24, seed=5, guiding=abi.GUIDE_ON))]
+        r.set_tuning(use_graph=0)
+        want = [img for img, _ in r.render_batch(jobs)]
+        want_g = r.render_batch(gjobs)[0][0]
+        r.set_tuning(use_graph=1)
+        for batch, ref in ((jobs, want), (gjobs, [want_g])):
+            for rep in range(3):
+                got = [img for img, _ in r.render_batch(batch)]
+                for g, w_ in zip(got, ref):
+                    assert np.array_equal(g, w_), rep
+                r.primary_hits(a, abi.render_opts(32, 32, 1, 8, seed=rep))  # eager launches between replays
