@@ -31,6 +31,8 @@ WORKLOADS = {
     "mesh": dict(scene="mesh", width=1920, height=1080, samples=256, max_depth=12, views=4, count_spp=8),
     # dispersive prism caustics: equal-RMSE time of the guided and the plain method
     "prism": dict(scene="prism", width=1024, height=1024, samples=512, max_depth=24, views=1, count_spp=0),
+    # 64 seeded scenes of uneven size, resolution and sample count, guided, in one batch
+    "batch64": dict(scene="uneven_batch", count=64, max_depth=12),
 }
 MFMA_FP16_PEAK_TFLOPS = 2500.0
 # network FLOPs per evaluation: 2 x (64x64 x 3 hidden + 40x64 output) multiply-adds
@@ -225,20 +227,53 @@ def main():
         del a, b
 
     w = dict(WORKLOADS[args.workload])
-    if args.spp:
-        w["samples"] = args.spp
-        w["count_spp"] = min(w["count_spp"], args.spp)
-    scene = getattr(synth, w["scene"])()
-    log("workload", args.workload, w)
     r = clovers_amd.Renderer(0)
-    t0 = time.perf_counter()
-    sid = r.upload(scene)
-    log("scene upload %.1f ms (host build %.1f ms)" % ((time.perf_counter() - t0) * 1e3, r.counters()["build_ms"]))
+    scene = None
+    if args.workload == "batch64":
+        # uneven batch: 64 seeded scenes of different size, resolution and samples, guided, one batch
+        t0 = time.perf_counter()
+        batch = synth.uneven_batch(w["count"], seed=1)
+        log("generated %d scenes in %.1f s" % (len(batch), time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        sids, build_ms = [], 0.0
+        for sc, _, _, _ in batch:
+            sids.append(r.upload(sc))
+            build_ms += r.counters()["build_ms"]
+        log("uploaded in %.1f s (host builds %.1f s)" % (time.perf_counter() - t0, build_ms / 1e3))
 
-    def opts_fn(spp=w["samples"], flags=0):
-        return lambda: abi.render_opts(w["width"], w["height"], spp, w["max_depth"], seed=1, flags=flags)
+        def jobs_for(flags=0, count=False):
+            return [(sid, abi.render_opts(side, side, max(1, spp // 16) if count else spp, w["max_depth"],
+                                          seed=k + 1, guiding=abi.GUIDE_ON, flags=flags))
+                    for k, (sid, (_, _, side, spp)) in enumerate(zip(sids, batch))]
+        samples = sum(side * side * spp for _, _, side, spp in batch)
+        wl_name = "uneven batch of %d scenes (%d-%d triangles, 0-200 spheres), %d-%d px a side, %d-%d spp, " \
+                  "depth %d, guided" % (len(batch), min(b[1] for b in batch), max(b[1] for b in batch),
+                                        min(b[2] for b in batch), max(b[2] for b in batch),
+                                        min(b[3] for b in batch), max(b[3] for b in batch), w["max_depth"])
+        wcfg = {"jobs": len(batch), "triangles": sum(b[1] for b in batch), "pixels": sum(b[2] ** 2 for b in batch),
+                "max_depth": w["max_depth"], "guiding": True}
+    else:
+        if args.spp:
+            w["samples"] = args.spp
+            w["count_spp"] = min(w["count_spp"], args.spp)
+        scene = getattr(synth, w["scene"])()
+        log("workload", args.workload, w)
+        t0 = time.perf_counter()
+        sid = r.upload(scene)
+        log("scene upload %.1f ms (host build %.1f ms)" % ((time.perf_counter() - t0) * 1e3,
+                                                          r.counters()["build_ms"]))
 
-    jobs = view_jobs(sid, w, opts_fn())
+        def jobs_for(flags=0, count=False):
+            spp = w["count_spp"] if count else w["samples"]
+            return view_jobs(sid, w, lambda: abi.render_opts(w["width"], w["height"], spp, w["max_depth"], seed=1,
+                                                             flags=flags))
+        samples = w["width"] * w["height"] * w["samples"] * w["views"]
+        wl_name = "%s %dx%d %dspp depth%d%s" % (w["scene"], w["width"], w["height"], w["samples"], w["max_depth"],
+                                                " x%d views" % w["views"] if w["views"] > 1 else "")
+        wcfg = {"views": w["views"], "width": w["width"], "height": w["height"], "spp": w["samples"],
+                "max_depth": w["max_depth"]}
+
+    jobs = jobs_for()
     tuning, tune_ms = r.tuning(), 0.0
     if not args.no_autotune:
         t0 = time.perf_counter()
@@ -256,7 +291,6 @@ def main():
         log("step %d: %.1f ms device" % (i, gpu_ms[-1]))
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
-    samples = w["width"] * w["height"] * w["samples"] * w["views"]
     value = samples / (ms_per_step / 1e3) / 1e6
     # The reference's estimator lets a NaN hero-only sample through (draw_cpu.rs:131-133; thin-film and
     # refraction take sqrt(1 - c^2) unclamped, thin_film.rs:45-48, materials.rs:194-200), so a head-on
@@ -265,7 +299,7 @@ def main():
     log("non-finite pixels: %d" % nonfinite)
 
     # dominant kernel: traversal.  Per-launch durations from HIP events ...
-    r.render_batch(view_jobs(sid, w, opts_fn(flags=abi.FLAG_TIME_KERNELS)))
+    r.render_batch(jobs_for(flags=abi.FLAG_TIME_KERNELS))
     ct = r.counters()
     avg_launch_ms = ct["extend_ms"] / ct["extend_launches"]
     rays_per_launch = ct["rays"] / ct["extend_launches"]
@@ -273,7 +307,7 @@ def main():
         % (ct["extend_launches"], ct["extend_ms"], avg_launch_ms, rays_per_launch))
     # ... and algorithmic bytes per ray from the counting traversal (the same
     # paths; for the mesh a prefix of the samples: the per-ray means converge)
-    r.render_batch(view_jobs(sid, w, opts_fn(spp=w["count_spp"], flags=abi.FLAG_COUNT)))
+    r.render_batch(jobs_for(flags=abi.FLAG_COUNT, count=True))
     cc = r.counters()
     per_ray = (cc["node_visits"] * cc["node_bytes"] + cc["prim_tests"] * cc["prim_bytes"]) / cc["rays"] \
         + RAY_STREAM_BYTES
@@ -286,6 +320,9 @@ def main():
                 "VALUUtilization in profiles/)") if args.workload != "mesh" else (
                 "the mesh BVH (40 MB of nodes) and triangles (64 MB) exceed the LDS window and L2; node and "
                 "triangle fetches come from HBM / the MALL (FETCH_SIZE in profiles/)")
+    if args.workload == "batch64":
+        lds_note = ("per-ray figures from a counting render at 1/16 of each job's samples over all 64 scenes; "
+                    "record bytes are those of the job with the most samples")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "%s (4-wide BVH traversal + primitive tests)" % TRACE_KERNELS[tuning["trace_kernel"]],
@@ -300,7 +337,7 @@ def main():
     r.close()
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and scene is not None:
         log("cpu baseline ...")
         cpu = cpu_baseline(scene, w, args.cpu_seconds)
         log("cpu baseline %.3f Msamples/s on %d threads" % (cpu["value"], cpu["cores"]))
@@ -309,15 +346,11 @@ def main():
         "metric": "Msamples/s at 1 GPU", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "%s %dx%d %dspp depth%d%s" % (w["scene"], w["width"], w["height"], w["samples"],
-                                                               w["max_depth"],
-                                                               " x%d views" % w["views"] if w["views"] > 1 else ""),
-                   "views": w["views"],
-                   "width": w["width"], "height": w["height"], "spp": w["samples"], "max_depth": w["max_depth"],
+        "config": dict({"workload": wl_name}, **wcfg, **{
                    "samples_per_step": samples, "nonfinite_pixels": nonfinite, "device_ms_per_step": round(float(np.mean(gpu_ms)), 3),
                    "parallelism": "1 GPU, wavefront",
                    "tuning": dict(tuning, trace_kernel=abi.TRACE_NAMES[tuning["trace_kernel"]]),
-                   "autotune_ms": round(tune_ms, 1)},
+                   "autotune_ms": round(tune_ms, 1)}),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,

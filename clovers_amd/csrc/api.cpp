@@ -352,9 +352,12 @@ DevJob make_job(const SceneRes *s, const clv_camera_init &cam, const clv_render_
 }
 
 /* Split the jobs' samples into wavefront passes of about `target` paths: a
- * pass holds, in job order, as many whole-sample slices (every pixel of a job,
- * `samples` samples each) as fit; a job larger than the target still gets one
- * sample per pass. */
+ * pass takes whole-sample slices (every pixel of a job, `samples` samples
+ * each) from the unfinished jobs in turn, starting after the job that ended
+ * the previous pass, until the target is reached; a job larger than the
+ * target still gets one sample per pass.  A guided job's first `rounds`
+ * slices hold at most `guided_cap` samples and mark their pass for a
+ * training round. */
 struct PassPlan {
   std::vector<DevItem> items;       /* all passes, concatenated */
   std::vector<uint32_t> first;      /* per pass: first item */
@@ -369,17 +372,20 @@ void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &s
                  uint32_t guided_cap, uint32_t rounds, PassPlan *pl) {
   const uint32_t n = (uint32_t)jobs.size();
   std::vector<uint32_t> done(n, 0), trained(n, 0);
-  uint32_t j = 0;
-  while (j < n) {
+  uint32_t remaining = n, cursor = 0;
+  while (remaining) {
     uint32_t budget = target, paths = 0, res = 0, depth = 0;
     bool train = false;
     pl->first.push_back((uint32_t)pl->items.size());
-    while (j < n) {
+    uint32_t last = cursor;
+    for (uint32_t step = 0; step < n && budget > 0; step++) {
+      const uint32_t j = (cursor + step) % n;
+      if (done[j] == jobs[j].spp) continue;
       const uint32_t pix = jobs[j].width * jobs[j].height;
       const uint32_t left = jobs[j].spp - done[j];
       uint32_t kk = budget / pix;
       if (kk == 0) {
-        if (paths) break;  /* start a new pass */
+        if (paths) continue; /* a smaller job may still fit */
         kk = 1;
       }
       if (kk > left) kk = left;
@@ -403,9 +409,11 @@ void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &s
         trained[j]++;
         train = true;
       }
-      if (done[j] == jobs[j].spp) j++;
-      if (budget == 0 || training) break; /* a training slice ends its pass */
+      if (done[j] == jobs[j].spp) remaining--;
+      last = j;
     }
+    /* the next pass starts after the last job served, so every job advances */
+    cursor = (last + 1) % n;
     pl->paths.push_back(paths);
     pl->res.push_back(res);
     pl->depth.push_back(depth);

@@ -326,6 +326,82 @@ def mesh(freq=224, seed=1, view=0):
     return s
 
 
+# ------------------------------------------------------------ uneven batch
+def batch_scene(seed, n_tris=None, n_spheres=None):
+    """One scene of the uneven batch: a noise-displaced geodesic icosphere of
+    about `n_tris` triangles (20 f^2 for the nearest frequency f >= 7) with a
+    random material (Lambertian, metal, dielectric or thin-film dielectric),
+    `n_spheres` random spheres with the sphere field's material mix, a grey
+    floor, a quad light (the MIS target) and a sky.  Seeded; returns
+    (scene, triangles)."""
+    r = _rng(10_000 + seed)
+    U = lambda: float(r.random())
+    if n_tris is None:
+        n_tris = int(round(math.exp(r.uniform(math.log(1e3), math.log(5e5)))))
+    if n_spheres is None:
+        n_spheres = int(r.integers(0, 201))
+    freq = max(7, int(round(math.sqrt(n_tris / 20.0))))
+    kind = int(r.integers(0, 4))
+    col = [0.2 + 0.7 * U(), 0.2 + 0.7 * U(), 0.2 + 0.7 * U()]
+    blob = [{"name": "blob", "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": col}},
+            {"name": "blob", "kind": "Metal", "fuzz": 0.3 * U(), "albedo": {"kind": "SolidColor", "color": col}},
+            {"name": "blob", "kind": "Dielectric", "refractive_index": 1.5},
+            {"name": "blob", "kind": "Dielectric", "refractive_index": 1.5,
+             "thin_film": {"refractive_index": 1.33, "thickness": float(r.uniform(250.0, 400.0))}}][kind]
+    mats = [blob,
+            {"name": "floor", "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": [0.5, 0.5, 0.5]}},
+            {"name": "lamp", "kind": "DiffuseLight", "emit": {"kind": "IlluminantD65", "intensity": 0.1}},
+            {"name": "glass", "kind": "Dielectric", "refractive_index": 1.5}]
+    objs = [{"kind": "Quad", "q": [-20.0, -0.2, -20.0], "u": [0.0, 0.0, 40.0], "v": [40.0, 0.0, 0.0],
+             "material": "floor"},
+            {"kind": "Quad", "priority": True, "q": [-1.0, 4.0, -1.0], "u": [2.0, 0.0, 0.0], "v": [0.0, 0.0, 2.0],
+             "material": "lamp"}]
+    for k in range(n_spheres):
+        ang, rad = 2 * math.pi * U(), 1.4 + 3.0 * U()
+        c = [rad * math.cos(ang), 0.0, rad * math.sin(ang)]
+        rr = 0.05 + 0.15 * U()
+        c[1] = -0.2 + rr
+        choose = U()
+        name = "s%d" % k
+        if choose < 0.8:
+            mats.append({"name": name, "kind": "Lambertian",
+                         "albedo": {"kind": "SolidColor", "color": [U() * U(), U() * U(), U() * U()]}})
+        elif choose < 0.95:
+            mats.append({"name": name, "kind": "Metal", "fuzz": 0.5 * U(),
+                         "albedo": {"kind": "SolidColor", "color": [0.5 + 0.5 * U(), 0.5 + 0.5 * U(), 0.5 + 0.5 * U()]}})
+        else:
+            name = "glass"
+        objs.append({"kind": "Sphere", "center": c, "radius": rr, "material": name})
+    az = 2 * math.pi * U()
+    sf = {
+        "time_0": 0, "time_1": 1,
+        "camera": {"look_from": [4.2 * math.sin(az), 1.4, 4.2 * math.cos(az)], "look_at": [0.0, 0.9, 0.0],
+                   "up": [0.0, 1.0, 0.0], "vertical_fov": 45.0, "aperture": 0.0, "focus_distance": 10.0},
+        "background_color": [0.7, 0.7, 0.9],
+        "objects": objs,
+        "materials": mats,
+    }
+    s = load_scene(sf)
+    P, T = displaced_icosphere(freq=freq, seed=seed)
+    tri = np.concatenate([P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]], axis=1)
+    s.add_objects(abi.OBJ_TRIANGLE, tri, 0)
+    return s, int(T.shape[0])
+
+
+def uneven_batch(count=64, seed=1):
+    """The uneven batch: `count` seeded scenes (batch_scene(seed + k)) with
+    square resolutions log-uniform in 256..1024 pixels a side and samples
+    per pixel log-uniform in 32..512.  Returns [(scene, triangles, side, spp)]."""
+    r = _rng(20_000 + seed)
+    out = []
+    for k in range(count):
+        side = int(round(math.exp(r.uniform(math.log(256), math.log(1024)))))
+        spp = int(round(math.exp(r.uniform(math.log(32), math.log(512)))))
+        scene, tris = batch_scene(seed + k)
+        out.append((scene, tris, side, spp))
+    return out
+
+
 SCENES = {"cornell": cornell, "spheres": spheres, "prism": prism, "mesh": mesh}
 
 # The configurations the measurements are quoted on: (scene, W, H, spp, max_depth)
@@ -334,4 +410,5 @@ CONFIGS = {
     "spheres": dict(scene="spheres", width=1920, height=1080, samples=64, max_depth=16),
     "mesh": dict(scene="mesh", width=1920, height=1080, samples=256, max_depth=12, views=4),
     "prism": dict(scene="prism", width=1024, height=1024, samples=0, max_depth=24),
+    "batch64": dict(scene="uneven_batch", count=64, max_depth=12),
 }
