@@ -46,6 +46,21 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, kernel, target_paths):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE and WRITE_SIZE passes of the same workload (profiles/traffic.json,
+    each counter scaled by what it reads for a 1 GiB copy), if they profiled
+    the same kernel; else None"""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    if not t["kernel"].startswith(kernel) or t.get("target_paths") != target_paths:
+        return None
+    return int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"])
+
+
 def view_jobs(sid, w, opts_fn):
     """the workload's jobs: view 0 with the scene camera, views 1.. with the
     other camera azimuths of the mesh scene"""
@@ -135,6 +150,7 @@ def run_prism(args):
     if args.guide_rounds:
         r.set_tuning(guide_rounds=args.guide_rounds)
     tuning = r.autotune([(sid, abi.render_opts(W, H, spps[-1], D, seed=1))])
+    r.set_tuning(use_graph=0)  # every render here is a different one: nothing to replay
     log("autotune: %s" % tuning)
     t0 = time.perf_counter()
     ref, _ = r.render(sid, abi.render_opts(W, H, args.ref_spp, D, seed=99, fp64_accum=1))
@@ -192,6 +208,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--target-paths", type=int, default=0,
+                    help="fix the pass size instead of autotuning (profiling the tuner's choice)")
+    ap.add_argument("--trace-kernel", default="", choices=["", "simple", "persistent", "ww", "lds_cached"],
+                    help="fix the traversal kernel instead of autotuning")
     ap.add_argument("--ref-spp", type=int, default=65536, help="prism workload: samples of the reference image")
     ap.add_argument("--guide-alpha", type=float, default=0.0, help="prism workload: override the guide's share")
     ap.add_argument("--guide-rounds", type=int, default=0, help="prism workload: override the training rounds")
@@ -274,6 +294,14 @@ def main():
                 "max_depth": w["max_depth"]}
 
     jobs = jobs_for()
+    if args.target_paths or args.trace_kernel:
+        fixed = {}
+        if args.target_paths:
+            fixed["target_paths"] = args.target_paths
+        if args.trace_kernel:
+            fixed["trace_kernel"] = {v: k for k, v in abi.TRACE_NAMES.items()}[args.trace_kernel]
+        r.set_tuning(**fixed)
+        args.no_autotune = True
     tuning, tune_ms = r.tuning(), 0.0
     if not args.no_autotune:
         t0 = time.perf_counter()
@@ -324,7 +352,9 @@ def main():
         lds_note = ("per-ray figures from a counting render at 1/16 of each job's samples over all 64 scenes; "
                     "record bytes are those of the job with the most samples")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.workload, TRACE_KERNELS[tuning["trace_kernel"]],
+                                            tuning["target_paths"]),
                 "kernel": "%s (4-wide BVH traversal + primitive tests)" % TRACE_KERNELS[tuning["trace_kernel"]],
                 "bytes_per_launch": int(per_launch), "mean_launch_ms": round(avg_launch_ms, 4),
                 "per_ray": {"node_visits": round(cc["node_visits"] / cc["rays"], 3),
@@ -333,7 +363,8 @@ def main():
                             "stream_bytes": RAY_STREAM_BYTES},
                 "rays_per_sample": round(ct["rays"] / samples, 4),
                 "note": "algorithmic bytes = counted node visits x 128 B + primitive tests x record bytes + 48 B "
-                        "of ray stream per ray; " + lds_note}
+                        "of ray stream per ray; traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the same "
+                        "workload and kernel (profiles/traffic.json); " + lds_note}
     r.close()
 
     cpu = None

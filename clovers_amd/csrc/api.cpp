@@ -135,6 +135,7 @@ struct clv_ctx {
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
   std::string gsig;
+  std::string last_sig;     /* signature of the previous eager render: a repeat is captured */
   uint32_t g_stamps = 0, g_t_after = 0, g_rounds = 0;
   bool g_trained_after = false;
 };
@@ -714,7 +715,11 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     for (auto &a : c->arena) add(&a.p, sizeof(a.p));
   }
   HIPCHK(c, hipEventRecord(c->events[0], c->stream));
-  if (use_graph && c->gexec && c->gsig == sig) {
+  /* a render is captured the second time its signature comes in a row, and
+   * replayed from then on; one-off renders stay eager */
+  const bool repeat = use_graph && !sig.empty() && (sig == c->last_sig || sig == c->gsig);
+  if (use_graph) c->last_sig = sig;
+  if (repeat && c->gexec && c->gsig == sig) {
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     gs.stamp += c->g_stamps;
     gs.t = c->g_t_after;
@@ -723,7 +728,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   } else {
     bool capturing = false;
     const uint32_t stamp0 = gs.stamp;
-    if (use_graph) {
+    if (repeat) {
       if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
       c->gexec = nullptr;
       c->gsig.clear();
@@ -862,6 +867,7 @@ clv_status clv_create(int hip_device, clv_ctx **out) {
   }
   if (const char *e = getenv("CLV_TARGET_PATHS")) c->target_paths = (uint32_t)strtoul(e, nullptr, 10);
   if (const char *e = getenv("CLV_GRID_CAP")) c->grid_cap = (uint32_t)strtoul(e, nullptr, 10);
+  if (const char *e = getenv("CLV_GRAPH")) c->use_graph = strtoul(e, nullptr, 10) != 0;
   if (c->target_paths < 65536) c->target_paths = 65536;
   if (c->grid_cap < 1) c->grid_cap = 1;
   *out = c;
@@ -1219,6 +1225,8 @@ clv_status clv_autotune(clv_ctx *c, const clv_job *jobs, uint32_t n, uint64_t tr
   clv_tuning best = start;
   double best_ms = 1e300;
   bool warmed = false;
+  const bool graph0 = c->use_graph;
+  c->use_graph = false; /* trials never repeat: no capture */
   for (const Cand &cd : cands) {
     clv_tuning t = start;
     t.target_paths = cd.target;
@@ -1229,11 +1237,17 @@ clv_status clv_autotune(clv_ctx *c, const clv_job *jobs, uint32_t n, uint64_t tr
     if (st == CLV_ENOMEM) continue; /* too large a pass for the device: skip */
     if (st != CLV_OK) {
       (void)clv_tuning_set(c, &start);
+      c->use_graph = graph0;
       return st;
     }
     if (!warmed) { /* first trial also pays allocation and first-launch costs: repeat it */
       warmed = true;
-      CHK(render_jobs(c, trial.data(), n));
+      st = render_jobs(c, trial.data(), n);
+      if (st != CLV_OK) {
+        (void)clv_tuning_set(c, &start);
+        c->use_graph = graph0;
+        return st;
+      }
     }
     const double ms = c->counters.render_ms;
     if (ms < best_ms) {
@@ -1241,6 +1255,7 @@ clv_status clv_autotune(clv_ctx *c, const clv_job *jobs, uint32_t n, uint64_t tr
       best = t;
     }
   }
+  best.use_graph = graph0 ? 1u : 0u;
   CHK(clv_tuning_set(c, &best));
   if (best_out) *best_out = best;
   return CLV_OK;

@@ -272,8 +272,9 @@ typedef struct {
   float guide_wclip;        /* training weight F/p capped at this multiple of its mean (default 1e6) */
   uint32_t guide_rounds;    /* training rounds per guided render (default 16); later samples use the
                                trained network in large passes without training */
-  uint32_t use_graph;       /* 1 (default): capture a render's launches as a hipGraph and replay it for an
-                               identical render; never for CLV_FLAG_TIME_KERNELS or guided CLV_FLAG_GUIDE_KEEP */
+  uint32_t use_graph;       /* 1 (default): a render repeated with identical arguments is captured as a hipGraph
+                               on its second call and replayed from then on; never for CLV_FLAG_TIME_KERNELS
+                               or guided CLV_FLAG_GUIDE_KEEP */
   uint32_t _pad;
 } clv_tuning;
 clv_status clv_tuning_get(const clv_ctx *ctx, clv_tuning *out);
