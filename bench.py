@@ -149,6 +149,8 @@ def run_prism(args):
         r.set_tuning(guide_alpha=args.guide_alpha)
     if args.guide_rounds:
         r.set_tuning(guide_rounds=args.guide_rounds)
+    if args.guide_train_spp:
+        r.set_tuning(guide_train_spp=args.guide_train_spp)
     tuning = r.autotune([(sid, abi.render_opts(W, H, spps[-1], D, seed=1))])
     r.set_tuning(use_graph=0)  # every render here is a different one: nothing to replay
     log("autotune: %s" % tuning)
@@ -215,6 +217,8 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=65536, help="prism workload: samples of the reference image")
     ap.add_argument("--guide-alpha", type=float, default=0.0, help="prism workload: override the guide's share")
     ap.add_argument("--guide-rounds", type=int, default=0, help="prism workload: override the training rounds")
+    ap.add_argument("--guide-train-spp", type=int, default=0,
+                    help="prism workload: override the samples per pixel between training rounds")
     ap.add_argument("--spp", type=int, default=0,
                     help="override the workload's samples per pixel (profiling runs: per-launch figures do not "
                          "depend on it; the JSON line then names the reduced workload)")

@@ -69,6 +69,8 @@ def load_library(path=LIB_PATH):
     lib.clv_guide_eval.argtypes = [vp, vp, C.c_uint32, vp, vp]
     lib.clv_guide_grad.restype = C.c_int
     lib.clv_guide_grad.argtypes = [vp, vp, C.c_uint32, vp]
+    lib.clv_plan_passes.restype = C.c_int
+    lib.clv_plan_passes.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.clv_primary_hits.restype = C.c_int
     lib.clv_primary_hits.argtypes = [vp, C.c_uint32, vp, vp, vp]
     lib.clv_counters_get.restype = C.c_int
@@ -247,6 +249,29 @@ class Renderer:
         c = abi.Counters()
         self._check(self.lib.clv_counters_get(self.h, C.byref(c)))
         return {k: getattr(c, k) for k, _ in abi.Counters._fields_ if k != "_pad"}
+
+
+def plan_passes(opts_list, target_paths=1 << 22, guide_train_spp=4, guide_rounds=16):
+    """clv_plan_passes (host only): the wavefront passes clv_render_batch would
+    run for jobs with these render options -> list of dicts (pass, job,
+    samples, sample_base, pass_paths, train)."""
+    lib = load_library()
+    jobs = (abi.Job * len(opts_list))()
+    for k, o in enumerate(opts_list):
+        jobs[k].opts = o
+    t = abi.Tuning()
+    t.target_paths, t.guide_train_spp, t.guide_rounds = target_paths, guide_train_spp, guide_rounds
+    n_items, n_passes = C.c_uint32(), C.c_uint32()
+    st = lib.clv_plan_passes(jobs, len(opts_list), C.byref(t), None, 0, C.byref(n_items), C.byref(n_passes))
+    if st != abi.OK:
+        raise ClvError(st, "clv_plan_passes rejected the jobs")
+    items = (abi.PassItem * max(1, n_items.value))()
+    st = lib.clv_plan_passes(jobs, len(opts_list), C.byref(t), items, n_items.value, C.byref(n_items),
+                             C.byref(n_passes))
+    if st != abi.OK:
+        raise ClvError(st, "clv_plan_passes rejected the jobs")
+    return [{"pass": it.pass_, "job": it.job, "samples": it.samples, "sample_base": it.sample_base,
+             "pass_paths": it.pass_paths, "train": it.train} for it in items[:n_items.value]]
 
 
 def draw(scene, width, height, samples, max_depth, seed=1, device=0, **kw):

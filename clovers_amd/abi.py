@@ -89,6 +89,11 @@ class Tuning(C.Structure):
                 ("guide_rounds", C.c_uint32), ("use_graph", C.c_uint32), ("_pad", C.c_uint32)]
 
 
+class PassItem(C.Structure):
+    _fields_ = [("pass_", C.c_uint32), ("job", C.c_uint32), ("samples", C.c_uint32), ("sample_base", C.c_uint32),
+                ("pass_paths", C.c_uint32), ("train", C.c_uint32)]
+
+
 class Counters(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("prim_tests", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),

@@ -352,78 +352,6 @@ DevJob make_job(const SceneRes *s, const clv_camera_init &cam, const clv_render_
   return j;
 }
 
-/* Split the jobs' samples into wavefront passes of about `target` paths: a
- * pass takes whole-sample slices (every pixel of a job, `samples` samples
- * each) from the unfinished jobs in turn, starting after the job that ended
- * the previous pass, until the target is reached; a job larger than the
- * target still gets one sample per pass.  A guided job's first `rounds`
- * slices hold at most `guided_cap` samples and mark their pass for a
- * training round. */
-struct PassPlan {
-  std::vector<DevItem> items;       /* all passes, concatenated */
-  std::vector<uint32_t> first;      /* per pass: first item */
-  std::vector<uint32_t> paths;      /* per pass: paths */
-  std::vector<uint32_t> res;        /* per pass: resolve threads */
-  std::vector<uint32_t> depth;      /* per pass: largest max_depth of its jobs */
-  std::vector<uint8_t> train;       /* per pass: holds a training slice of a guided job */
-  uint32_t max_paths = 0;
-};
-
-void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &sample_offset, uint32_t target,
-                 uint32_t guided_cap, uint32_t rounds, PassPlan *pl) {
-  const uint32_t n = (uint32_t)jobs.size();
-  std::vector<uint32_t> done(n, 0), trained(n, 0);
-  uint32_t remaining = n, cursor = 0;
-  while (remaining) {
-    uint32_t budget = target, paths = 0, res = 0, depth = 0;
-    bool train = false;
-    pl->first.push_back((uint32_t)pl->items.size());
-    uint32_t last = cursor;
-    for (uint32_t step = 0; step < n && budget > 0; step++) {
-      const uint32_t j = (cursor + step) % n;
-      if (done[j] == jobs[j].spp) continue;
-      const uint32_t pix = jobs[j].width * jobs[j].height;
-      const uint32_t left = jobs[j].spp - done[j];
-      uint32_t kk = budget / pix;
-      if (kk == 0) {
-        if (paths) continue; /* a smaller job may still fit */
-        kk = 1;
-      }
-      if (kk > left) kk = left;
-      const bool training = jobs[j].guiding && trained[j] < rounds;
-      if (training && kk > guided_cap) kk = guided_cap; /* a training round after every such slice */
-      DevItem it;
-      memset(&it, 0, sizeof(it));
-      it.job = j;
-      it.path_base = paths;
-      it.pixels = pix;
-      it.samples = kk;
-      it.sample_base = sample_offset[j] + done[j];
-      it.res_base = res;
-      pl->items.push_back(it);
-      paths += kk * pix;
-      res += pix;
-      depth = std::max(depth, jobs[j].max_depth);
-      budget = budget > kk * pix ? budget - kk * pix : 0;
-      done[j] += kk;
-      if (training) {
-        trained[j]++;
-        train = true;
-      }
-      if (done[j] == jobs[j].spp) remaining--;
-      last = j;
-    }
-    /* the next pass starts after the last job served, so every job advances */
-    cursor = (last + 1) % n;
-    pl->paths.push_back(paths);
-    pl->res.push_back(res);
-    pl->depth.push_back(depth);
-    pl->train.push_back(train ? 1 : 0);
-    pl->max_paths = std::max(pl->max_paths, paths);
-  }
-  pl->first.push_back((uint32_t)pl->items.size());
-}
-
 /* ------------------------------------------------------------ guiding --- */
 clv_status guide_ensure_net(clv_ctx *c) {
   GuideState &g = c->guide;
@@ -1040,6 +968,45 @@ clv_status clv_render(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, f
 clv_status clv_render_batch(clv_ctx *c, const clv_job *jobs, uint32_t n_jobs) {
   if (!c) return CLV_EINVAL;
   return render_jobs(c, jobs, n_jobs);
+}
+
+clv_status clv_plan_passes(const clv_job *jobs, uint32_t n, const clv_tuning *t, clv_pass_item *items, uint32_t cap,
+                           uint32_t *n_items, uint32_t *n_passes) {
+  if (!jobs || !t || !n_items || !n_passes || n == 0 || n > MAX_JOBS || (cap && !items)) return CLV_EINVAL;
+  if (t->target_paths < 1) return CLV_EINVAL;
+  std::vector<DevJob> dj(n);
+  std::vector<uint32_t> soff(n);
+  for (uint32_t k = 0; k < n; k++) {
+    const clv_render_opts &o = jobs[k].opts;
+    if (o.width == 0 || o.height == 0 || (uint64_t)o.width * o.height > (1ull << 27) || o.samples == 0 ||
+        o.max_depth > 1000)
+      return CLV_EINVAL;
+    memset(&dj[k], 0, sizeof(DevJob));
+    dj[k].width = o.width;
+    dj[k].height = o.height;
+    dj[k].spp = o.samples;
+    dj[k].max_depth = o.max_depth;
+    dj[k].guiding = o.guiding;
+    soff[k] = o.sample_offset;
+  }
+  PassPlan pl;
+  plan_passes(dj, soff, t->target_paths, std::max<uint32_t>(1, t->guide_train_spp), t->guide_rounds, &pl);
+  const uint32_t passes = (uint32_t)pl.paths.size();
+  uint32_t w = 0;
+  for (uint32_t p = 0; p < passes; p++)
+    for (uint32_t i = pl.first[p]; i < pl.first[p + 1]; i++, w++) {
+      if (w >= cap) continue;
+      const DevItem &it = pl.items[i];
+      items[w].pass = p;
+      items[w].job = it.job;
+      items[w].samples = it.samples;
+      items[w].sample_base = it.sample_base;
+      items[w].pass_paths = pl.paths[p];
+      items[w].train = pl.train[p];
+    }
+  *n_items = (uint32_t)pl.items.size();
+  *n_passes = passes;
+  return CLV_OK;
 }
 
 clv_status clv_tuning_get(const clv_ctx *c, clv_tuning *out) {

@@ -926,4 +926,60 @@ int rebase_host_scene(HostScene *hs, const ArenaBase &b, std::string *err) {
   return CLV_OK;
 }
 
+void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &sample_offset, uint32_t target,
+                 uint32_t guided_cap, uint32_t rounds, PassPlan *pl) {
+  const uint32_t n = (uint32_t)jobs.size();
+  std::vector<uint32_t> done(n, 0), trained(n, 0);
+  uint32_t remaining = n, cursor = 0;
+  while (remaining) {
+    uint32_t budget = target, paths = 0, res = 0, depth = 0;
+    bool train = false;
+    pl->first.push_back((uint32_t)pl->items.size());
+    uint32_t last = cursor;
+    for (uint32_t step = 0; step < n && budget > 0; step++) {
+      const uint32_t j = (cursor + step) % n;
+      if (done[j] == jobs[j].spp) continue;
+      const uint32_t pix = jobs[j].width * jobs[j].height;
+      const uint32_t left = jobs[j].spp - done[j];
+      uint32_t kk = budget / pix;
+      if (kk == 0) {
+        if (paths) continue; /* a smaller job may still fit */
+        kk = 1;
+      }
+      if (kk > left) kk = left;
+      const bool training = jobs[j].guiding && trained[j] < rounds;
+      if (training && kk > guided_cap) kk = guided_cap; /* a training round after every such slice */
+      DevItem it;
+      memset(&it, 0, sizeof(it));
+      it.job = j;
+      it.path_base = paths;
+      it.pixels = pix;
+      it.samples = kk;
+      it.sample_base = sample_offset[j] + done[j];
+      it.res_base = res;
+      pl->items.push_back(it);
+      paths += kk * pix;
+      res += pix;
+      depth = std::max(depth, jobs[j].max_depth);
+      budget = budget > kk * pix ? budget - kk * pix : 0;
+      done[j] += kk;
+      if (training) {
+        trained[j]++;
+        train = true;
+      }
+      if (done[j] == jobs[j].spp) remaining--;
+      last = j;
+    }
+    /* the next pass starts after the last job served, so every job advances */
+    cursor = (last + 1) % n;
+    pl->paths.push_back(paths);
+    pl->res.push_back(res);
+    pl->depth.push_back(depth);
+    pl->train.push_back(train ? 1 : 0);
+    pl->max_paths = std::max(pl->max_paths, paths);
+  }
+  pl->first.push_back((uint32_t)pl->items.size());
+}
+
+
 }  // namespace clv

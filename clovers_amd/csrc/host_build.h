@@ -45,6 +45,27 @@ struct ArenaBase {
  * offsets, light-tree words.  Fails if a leaf offset leaves 28 bits. */
 int rebase_host_scene(HostScene *hs, const ArenaBase &b, std::string *err);
 
+/* The wavefront passes of a batch (see plan_passes). */
+struct PassPlan {
+  std::vector<DevItem> items;       /* all passes, concatenated */
+  std::vector<uint32_t> first;      /* per pass: first item (one more entry: the end) */
+  std::vector<uint32_t> paths;      /* per pass: paths */
+  std::vector<uint32_t> res;        /* per pass: resolve threads */
+  std::vector<uint32_t> depth;      /* per pass: largest max_depth of its jobs */
+  std::vector<uint8_t> train;       /* per pass: holds a training slice of a guided job */
+  uint32_t max_paths = 0;
+};
+
+/* Split the jobs' samples into wavefront passes of about `target` paths: a
+ * pass takes whole-sample slices (every pixel of a job, `samples` samples
+ * each) from the unfinished jobs in turn, starting after the job that ended
+ * the previous pass, until the target is reached; a job larger than the
+ * target still gets one sample per pass.  A guided job's first `rounds`
+ * slices hold at most `guided_cap` samples and mark their pass for a
+ * training round.  Uses width, height, spp, max_depth, guiding of `jobs`. */
+void plan_passes(const std::vector<DevJob> &jobs, const std::vector<uint32_t> &sample_offset, uint32_t target,
+                 uint32_t guided_cap, uint32_t rounds, PassPlan *pl);
+
 /* Camera::new (camera.rs:59-99) for the given aspect ratio. */
 DevCamera make_camera(const clv_camera_init &ci, float aspect);
 

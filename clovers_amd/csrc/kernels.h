@@ -25,35 +25,6 @@ namespace clv {
 #define META_JOB_SHIFT 19u
 #define MAX_JOBS 8192u           /* 13 bits of job index */
 
-/* per-job constants of a batch: one view of one scene */
-struct DevJob {
-  DevCamera cam;
-  uint32_t root;          /* BVH root node (arena index) */
-  uint32_t light_root;    /* light-tree root word (arena indices) */
-  uint32_t light_base;    /* the scene's MIS leaves: lights[light_base, +n_lights) */
-  uint32_t n_lights;
-  int32_t bg_spd;         /* background SPD offset in spd[] */
-  uint32_t width, height;
-  uint32_t max_depth;
-  uint64_t seed;
-  uint32_t pixel_base;    /* first accumulator pixel of the job */
-  uint32_t spp;           /* samples per pixel of the whole job */
-  uint32_t fp64;          /* accumulate in f64 */
-  uint32_t guiding;        /* clv_guiding */
-  float lo[3], hi[3];      /* scene box: guiding inputs are positions normalised to it */
-};
-
-/* one job's share of a wavefront pass: `samples` samples of every pixel */
-struct DevItem {
-  uint32_t job;
-  uint32_t path_base;     /* first path slot of the item */
-  uint32_t pixels;        /* W*H of the job */
-  uint32_t samples;       /* samples per pixel in this pass */
-  uint32_t sample_base;   /* absolute index of the item's first sample */
-  uint32_t res_base;      /* first resolve thread of the item (prefix sum of pixels) */
-  uint32_t pad[2];
-};
-
 /* The scene arena: every uploaded scene lives in these arrays with global
  * indices (child words, record offsets, material / SPD / light indices are
  * rebased at upload), so one traversal launch serves rays of any scene; a ray

@@ -287,6 +287,22 @@ clv_status clv_tuning_set(clv_ctx *ctx, const clv_tuning *t);
 clv_status clv_autotune(clv_ctx *ctx, const clv_job *jobs, uint32_t n_jobs, uint64_t trial_samples,
                         clv_tuning *best_out);
 
+/* The pass plan clv_render_batch would use for `jobs` under `tuning`
+ * (target_paths, guide_train_spp, guide_rounds), computed on the host, no
+ * device or uploaded scene needed (scene ids are not looked up).  Items are
+ * in pass order; up to `cap` are written, *n_items and *n_passes receive the
+ * full counts. */
+typedef struct {
+  uint32_t pass;         /* pass index */
+  uint32_t job;          /* index into jobs */
+  uint32_t samples;      /* samples per pixel of the job in this pass */
+  uint32_t sample_base;  /* absolute index of the slice's first sample */
+  uint32_t pass_paths;   /* paths of the whole pass */
+  uint32_t train;        /* 1: a training round of the guiding network follows the pass */
+} clv_pass_item;
+clv_status clv_plan_passes(const clv_job *jobs, uint32_t n_jobs, const clv_tuning *tuning, clv_pass_item *items,
+                           uint32_t cap, uint32_t *n_items, uint32_t *n_passes);
+
 /* Guiding network probes (no reference counterpart: the learned proposal is
  * this library's own).  The network maps (position normalised to the scene
  * box, oriented normal, (lambda - 380) / 400) to 8 vMF lobes; its fp32

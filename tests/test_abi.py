@@ -40,7 +40,7 @@ int main(void) {
          sizeof(clv_xform), sizeof(clv_camera_init), sizeof(clv_scene_desc), sizeof(clv_render_opts),
          sizeof(clv_counters), sizeof(clv_scene_info), offsetof(clv_scene_desc, camera), sizeof(clv_job),
          offsetof(clv_job, opts), offsetof(clv_job, xyz_out));
-  printf("%zu\n", sizeof(clv_tuning));
+  printf("%zu %zu\n", sizeof(clv_tuning), sizeof(clv_pass_item));
   return 0;
 }
 '''
@@ -53,7 +53,7 @@ int main(void) {
     want = [abi.OBJECT_DTYPE.itemsize, abi.MATERIAL_DTYPE.itemsize, abi.XFORM_DTYPE.itemsize,
             C.sizeof(abi.CameraInit), C.sizeof(abi.SceneDesc), C.sizeof(abi.RenderOpts), C.sizeof(abi.Counters),
             C.sizeof(abi.SceneInfo), abi.SceneDesc.camera.offset, C.sizeof(abi.Job), abi.Job.opts.offset,
-            abi.Job.xyz_out.offset, C.sizeof(abi.Tuning)]
+            abi.Job.xyz_out.offset, C.sizeof(abi.Tuning), C.sizeof(abi.PassItem)]
     assert got == want
 
 
@@ -64,6 +64,7 @@ def test_no_device_calls_fail_cleanly_without_gpu():
     assert lib.clv_primary_hits(None, 1, None, None, None) == abi.EINVAL
     assert lib.clv_render_batch(None, None, 0) == abi.EINVAL
     assert lib.clv_autotune(None, None, 0, 0, None) == abi.EINVAL
+    assert lib.clv_plan_passes(None, 0, None, None, 0, None, None) == abi.EINVAL
     assert lib.clv_tuning_get(None, None) == abi.EINVAL
     assert lib.clv_tuning_set(None, None) == abi.EINVAL
     assert lib.clv_counters_get(None, None) == abi.EINVAL

@@ -1,0 +1,86 @@
+"""The wavefront pass planner (host code behind clv_render_batch), through the
+C ABI's clv_plan_passes; no device needed."""
+import random
+
+import pytest
+
+import clovers_amd
+from clovers_amd import abi
+
+
+def check_plan(opts_list, items, target):
+    n = len(opts_list)
+    next_sample = [o.sample_offset for o in opts_list]
+    passes = {}
+    for it in items:
+        o = opts_list[it["job"]]
+        # every job's samples are covered once, in sample order (each pixel sums its samples in order)
+        assert it["sample_base"] == next_sample[it["job"]]
+        next_sample[it["job"]] += it["samples"]
+        assert it["samples"] >= 1
+        passes.setdefault(it["pass"], []).append(it)
+    for k, o in enumerate(opts_list):
+        assert next_sample[k] == o.sample_offset + o.samples, k
+    for p, its in passes.items():
+        paths = sum(opts_list[i["job"]].width * opts_list[i["job"]].height * i["samples"] for i in its)
+        assert all(i["pass_paths"] == paths for i in its)
+        # within the target unless a single one-sample slice is larger
+        assert paths <= target or (len(its) == 1 and its[0]["samples"] == 1), (p, paths)
+        assert len({i["job"] for i in its}) == len(its)  # a job appears once per pass
+    assert sorted(passes) == list(range(len(passes)))
+    return passes
+
+
+def test_single_job_fills_passes():
+    o = [abi.render_opts(1920, 1080, 64, 16)]
+    items = clovers_amd.plan_passes(o, target_paths=1 << 24)
+    passes = check_plan(o, items, 1 << 24)
+    assert len(passes) == 8 and all(it["samples"] == 8 for it in items)
+
+
+def test_job_larger_than_the_target_gets_one_sample_per_pass():
+    o = [abi.render_opts(4096, 4096, 3, 4)]
+    items = clovers_amd.plan_passes(o, target_paths=1 << 20)
+    check_plan(o, items, 1 << 20)
+    assert [it["samples"] for it in items] == [1, 1, 1]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batches(seed):
+    rng = random.Random(seed)
+    o = [abi.render_opts(rng.randint(1, 700), rng.randint(1, 700), rng.randint(1, 300), rng.randint(0, 30),
+                         sample_offset=rng.randint(0, 50)) for _ in range(rng.randint(1, 40))]
+    target = rng.choice([1 << 16, 1 << 20, 1 << 22])
+    items = clovers_amd.plan_passes(o, target_paths=target)
+    passes = check_plan(o, items, target)
+    # passes are full except where no unfinished job's slice fits
+    assert all(it["train"] == 0 for it in items)
+    assert len(passes) >= 1
+
+
+def test_round_robin_serves_every_job():
+    """With many jobs larger than a pass, consecutive passes move on to the
+    next jobs instead of finishing one job first."""
+    o = [abi.render_opts(512, 512, 64, 8) for _ in range(8)]
+    items = clovers_amd.plan_passes(o, target_paths=4 * 512 * 512)
+    check_plan(o, items, 4 * 512 * 512)
+    first_passes = [it["job"] for it in items if it["pass"] < 8]
+    assert sorted(set(first_passes)) == list(range(8))
+
+
+def test_guided_jobs_train_in_short_slices_then_render_in_long_ones():
+    o = [abi.render_opts(256, 256, 200, 24, guiding=abi.GUIDE_ON), abi.render_opts(128, 128, 50, 8)]
+    items = clovers_amd.plan_passes(o, target_paths=1 << 24, guide_train_spp=4, guide_rounds=5)
+    check_plan(o, items, 1 << 24)
+    g = [it for it in items if it["job"] == 0]
+    assert [it["samples"] for it in g[:5]] == [4] * 5
+    assert all(it["train"] == 1 for it in g[:5])
+    assert sum(it["samples"] for it in g[5:]) == 180 and all(it["samples"] > 4 for it in g[5:])
+    assert not any(it["train"] for it in items if it["pass"] > g[4]["pass"])
+
+
+def test_invalid_jobs_are_rejected():
+    with pytest.raises(clovers_amd.ClvError):
+        clovers_amd.plan_passes([abi.render_opts(0, 10, 1, 1)])
+    with pytest.raises(clovers_amd.ClvError):
+        clovers_amd.plan_passes([abi.render_opts(10, 10, 0, 1)])
