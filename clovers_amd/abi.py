@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 NO_XFORM = 0xFFFFFFFF
 MISS = 0xFFFFFFFF
 
@@ -17,14 +17,14 @@ STATUS_NAMES = {0: "CLV_OK", 1: "CLV_EINVAL", 2: "CLV_ENOMEM", 3: "CLV_EDEVICE",
                 4: "CLV_EUNSUPPORTED", 5: "CLV_EINTERNAL"}
 
 # object kinds (objects.rs:59-87)
-OBJ_SPHERE, OBJ_QUAD, OBJ_TRIANGLE, OBJ_BOXY, OBJ_TRIANGLE_QUV = range(5)
+OBJ_SPHERE, OBJ_QUAD, OBJ_TRIANGLE, OBJ_BOXY, OBJ_TRIANGLE_QUV, OBJ_MOVING_SPHERE = range(6)
 # xform ops (objects/translate.rs, objects/rotate.rs)
 XF_TRANSLATE, XF_ROTATE_Y = 0, 1
 MAX_XFORM_OPS = 4
 # material kinds (materials.rs:141-156)
-MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DISPERSIVE, MAT_DIFFUSE_LIGHT = range(5)
+MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DISPERSIVE, MAT_DIFFUSE_LIGHT, MAT_CONE_LIGHT = range(6)
 # texture kinds (textures.rs:22-109)
-TEX_SOLID, TEX_ILLUMINANT_D65, TEX_ILLUMINANT_D50 = range(3)
+TEX_SOLID, TEX_ILLUMINANT_D65, TEX_ILLUMINANT_D50, TEX_SPATIAL_CHECKER, TEX_SURFACE_CHECKER = range(5)
 # BVH algorithms (bvh.rs:32-40)
 BVH_LAM, BVH_SAH = 0, 1
 GUIDE_OFF, GUIDE_ON = 0, 1
@@ -42,9 +42,10 @@ assert XFORM_DTYPE.itemsize == 80
 MATERIAL_DTYPE = np.dtype([
     ("kind", "<u4"), ("texture", "<u4"), ("color", "<f4", (3,)), ("intensity", "<f4"),
     ("fuzz", "<f4"), ("refractive_index", "<f4"), ("cauchy_a", "<f4"), ("cauchy_b", "<f4"),
-    ("thin_film", "<u4"), ("film_index", "<f4"), ("film_thickness", "<f4"), ("_pad", "<u4"),
+    ("thin_film", "<u4"), ("film_index", "<f4"), ("film_thickness", "<f4"), ("density", "<f4"),
+    ("odd_color", "<f4", (3,)), ("spread", "<f4"),
 ])
-assert MATERIAL_DTYPE.itemsize == 56, MATERIAL_DTYPE.itemsize
+assert MATERIAL_DTYPE.itemsize == 72, MATERIAL_DTYPE.itemsize
 
 
 class CameraInit(C.Structure):

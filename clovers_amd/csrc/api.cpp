@@ -55,6 +55,7 @@ struct SceneRes {
   uint32_t n_lights = 0;
   int32_t bg_spd = -1;
   bool has_xforms = false;
+  bool has_ext = false;      /* checker textures or cone lights */
   clv_camera_init camera;
   uint32_t n_nodes = 0, n_prims = 0, prim_bytes = 0, depth = 0;
   uint32_t prim_units = 0;   /* float4 units of primitive records */
@@ -455,7 +456,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   std::vector<const SceneRes *> sres(n);
   uint64_t pixels = 0, samples = 0;
   uint32_t flags = 0;
-  bool any_xform = false;
+  bool any_xform = false, any_ext = false;
   for (uint32_t k = 0; k < n; k++) {
     const clv_job &jb = jobs[k];
     CHK(check_opts(c, &jb.opts));
@@ -471,6 +472,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     samples += (uint64_t)jb.opts.width * jb.opts.height * jb.opts.samples;
     flags |= jb.opts.flags;
     any_xform = any_xform || it->second->has_xforms;
+    any_ext = any_ext || it->second->has_ext;
   }
   if (pixels > (1ull << 28)) return fail(c, CLV_EINVAL, "batch larger than 2^28 pixels");
   bool any_guide = false;
@@ -505,6 +507,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
                            c->stream));
   DevScene ds = arena_view(c);
   ds.has_xforms = any_xform ? 1u : 0u;
+  ds.has_ext = any_ext ? 1u : 0u;
   const bool timing = (flags & CLV_FLAG_TIME_KERNELS) != 0;
   const bool counting = (flags & CLV_FLAG_COUNT) != 0;
   uint32_t extends = 0;
@@ -628,8 +631,8 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     add(pl.res.data(), pl.res.size() * 4);
     add(pl.depth.data(), pl.depth.size() * 4);
     add(pl.train.data(), pl.train.size());
-    const uint32_t sc[8] = {flags, any_sq, any_guide, frozen, (uint32_t)c->trace_variant, c->grid_cap, ds.has_xforms,
-                            gs.train_spp};
+    const uint32_t sc[9] = {flags, any_sq, any_guide, frozen, (uint32_t)c->trace_variant, c->grid_cap, ds.has_xforms,
+                            gs.train_spp, ds.has_ext};
     add(sc, sizeof(sc));
     const float gf[4] = {gs.alpha, gs.lr, gs.wclip, (float)gs.records};
     add(gf, sizeof(gf));
@@ -880,6 +883,7 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
   s->light_root = hs.light_root;
   s->bg_spd = hs.background_spd;
   s->has_xforms = !hs.xforms.empty();
+  for (const DevMaterial &m : hs.materials) s->has_ext = s->has_ext || m.tex != TX_PLAIN || m.cone != 0u;
   s->camera = hs.camera;
   s->n_nodes = (uint32_t)hs.nodes.size();
   s->prim_units = (uint32_t)(hs.prims.size() / 4);
@@ -1249,6 +1253,7 @@ clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   HIPCHK(c, hipMemcpyAsync(c->items, &item, sizeof(item), hipMemcpyHostToDevice, c->stream));
   DevScene ds = arena_view(c);
   ds.has_xforms = it->second->has_xforms ? 1u : 0u;
+  ds.has_ext = it->second->has_ext ? 1u : 0u;
   GenParams gp;
   memset(&gp, 0, sizeof(gp));
   gp.items = c->items;

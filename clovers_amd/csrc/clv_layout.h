@@ -6,8 +6,9 @@
  *   nodes      BVH4 nodes, 128 B each, 128-B aligned: child boxes as SoA
  *              (lo/hi per axis x 4 children) + 4 child words.
  *   prims      variable-size primitive records in float4 units:
- *                sphere 2 x float4 (32 B), triangle 4 x float4 (64 B: corners,
- *                plane d, unit normal),
+ *                sphere 2 x float4 (32 B), moving sphere 3 x float4 (48 B:
+ *                centre at time 0, radius, centre_1 - centre_0), triangle
+ *                4 x float4 (64 B: corners, plane d, unit normal),
  *                quad 5 x float4 (80 B); word .w of float4 #0 is the header
  *              (type | (xform+1) << 2), word .w of float4 #1 the primitive
  *              index into prim_info.
@@ -30,11 +31,16 @@
 #define PT_SPHERE 0u
 #define PT_QUAD 1u
 #define PT_TRI 2u
+#define PT_MSPHERE 3u
 
 /* record sizes in float4 units */
 #define PREC_SPHERE 2u
 #define PREC_TRI 4u
 #define PREC_QUAD 5u
+#define PREC_MSPHERE 3u
+/* record size by type as a nibble table: PT_SPHERE 2, PT_QUAD 5, PT_TRI 4, PT_MSPHERE 3 */
+#define PREC_TABLE 0x3452u
+#define PREC_OF(type) ((PREC_TABLE >> ((type) << 2)) & 0xFu)
 
 /* BVH4 child word: interior node index, or a leaf, or empty */
 #define CHILD_LEAF 0x80000000u
@@ -75,18 +81,28 @@ typedef struct {
 #define MC_GLASS 4u
 #define MC_COUNT 5u
 
+/* how a material's texture picks its SPD per hit */
+#define TX_PLAIN 0u      /* one table: solid colour or illuminant */
+#define TX_SPATIAL 1u    /* checker by world position: odd cells where sin(dx)sin(dy)sin(dz) < 0 */
+#define TX_SURFACE 2u    /* checker by surface (u, v): odd cells where sin(du)sin(dv) < 0 */
+
 typedef struct {
   uint32_t kind;         /* clv_material_kind */
   uint32_t mclass;       /* MC_* */
-  int32_t albedo_spd;    /* offset of the reflectance SPD in spd[] (or -1) */
-  int32_t emit_spd;      /* offset of the emission SPD (or -1) */
+  int32_t albedo_spd;    /* offset of the reflectance SPD in spd[] (or -1); checkers: even cells */
+  int32_t emit_spd;      /* offset of the emission SPD (or -1); checkers: even cells */
   float fuzz;            /* metal */
   float ior;             /* dielectric refractive index */
   float cauchy_a, cauchy_b;
   uint32_t thin_film;
   float film_ior, film_thickness;
-  uint32_t pad;
-} DevMaterial;           /* 48 bytes */
+  uint32_t tex;          /* TX_* of the albedo (Lambertian/Metal) or emission (lights) texture */
+  int32_t odd_spd;       /* checkers: offset of the odd cells' SPD */
+  float density_pi;      /* checkers: density * pi (the reference's f32 product) */
+  float cone_cos;        /* ConeLight: c = -d.n / (|d| |n|) in [-1, 1] is outside the cone iff
+                            c < cone_cos (the f32 threshold of acos(c) > spread in radians) */
+  uint32_t cone;         /* 1: ConeLight */
+} DevMaterial;           /* 64 bytes */
 
 /* light (MIS target) record: the reference's Quad/Triangle/Sphere params */
 typedef struct {

@@ -30,6 +30,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <numeric>
 
@@ -220,6 +221,9 @@ void write_record(std::vector<float> &out, const FlatPrim &p) {
   if (p.type == PT_SPHERE) {        /* [c, hdr] [r, 0, 0, info] */
     float r[8] = {d[0], d[1], d[2], fh, d[3], z, z, fi};
     out.insert(out.end(), r, r + 8);
+  } else if (p.type == PT_MSPHERE) { /* [c0, hdr] [r, 0, 0, info] [c1 - c0, 0] */
+    float r[12] = {d[0], d[1], d[2], fh, d[3], z, z, fi, d[4], d[5], d[6], z};
+    out.insert(out.end(), r, r + 12);
   } else if (p.type == PT_TRI) {    /* [a, hdr] [b, info] [c, plane d] [n, 0] */
     float r[16] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], d[12], d[9], d[10], d[11], z};
     out.insert(out.end(), r, r + 16);
@@ -607,6 +611,37 @@ DevCamera make_camera(const clv_camera_init &ci, float aspect) {
   return c;
 }
 
+/* f32 values in their numeric order as integers (-0 and +0 both map to 0) */
+static int32_t f32_order(float x) {
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  return (b & 0x80000000u) ? -(int32_t)(b & 0x7FFFFFFFu) : (int32_t)b;
+}
+static float f32_from_order(int32_t k) {
+  uint32_t b = k < 0 ? 0x80000000u | (uint32_t)(-k) : (uint32_t)k;
+  float x;
+  memcpy(&x, &b, 4);
+  return x;
+}
+
+/* ConeLight::emit (cone_light.rs:40-53) is dark where acos(c) > spread, with
+ * c = -d.n / (|d| |n|).  acosf does not increase with c, so over c in
+ * [-1, 1] that set is {c < thr}; thr is the smallest f32 with acosf(thr) <=
+ * spread, found by bisection over the ordered f32 values with the same acosf
+ * the reference calls (f32::acos is the platform's acosf).  Outside [-1, 1]
+ * acosf is NaN and the comparison false: the device tests the range first. */
+float cone_threshold(float spread_rad) {
+  if (!(acosf(1.0f) <= spread_rad)) return INFINITY;   /* spread < 0: every c is outside */
+  if (acosf(-1.0f) <= spread_rad) return -1.0f;        /* spread >= pi: no c is outside */
+  int32_t lo = f32_order(-1.0f), hi = f32_order(1.0f); /* acosf(lo) > spread >= acosf(hi) */
+  while (hi - lo > 1) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if (acosf(f32_from_order(mid)) > spread_rad) lo = mid;
+    else hi = mid;
+  }
+  return f32_from_order(hi);
+}
+
 int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
   auto t0 = std::chrono::steady_clock::now();
   if (!d || d->abi_version != CLV_ABI_VERSION) { *err = "descriptor missing or ABI version mismatch"; return CLV_EINVAL; }
@@ -656,13 +691,28 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
     dm.thin_film = m.thin_film ? 1u : 0u;
     dm.film_ior = m.film_index;
     dm.film_thickness = m.film_thickness;
-    if (m.texture > CLV_TEX_ILLUMINANT_D50) { *err = "unknown texture kind"; return CLV_EINVAL; }
+    dm.tex = TX_PLAIN;
+    dm.odd_spd = -1;
+    if (m.texture > CLV_TEX_SURFACE_CHECKER) { *err = "unknown texture kind"; return CLV_EINVAL; }
     if (!finite_all(m.color, 3)) { *err = "material colour is not finite"; return CLV_EINVAL; }
+    const bool checker = m.texture == CLV_TEX_SPATIAL_CHECKER || m.texture == CLV_TEX_SURFACE_CHECKER;
+    const bool plain_color = m.texture == CLV_TEX_SOLID || checker;
+    if (checker && (!finite_all(m.odd_color, 3) || !std::isfinite(m.density))) {
+      *err = "checker parameters are not finite";
+      return CLV_EINVAL;
+    }
+    /* a checker texture of a material that reads its texture (albedo or emission) */
+    auto use_checker = [&]() {
+      dm.tex = m.texture == CLV_TEX_SPATIAL_CHECKER ? TX_SPATIAL : TX_SURFACE;
+      dm.density_pi = m.density * PI_F;  /* spatial_checker.rs:76, surface_checker.rs:74 */
+      dm.odd_spd = spd_of(m.odd_color);
+    };
     switch (m.kind) {
       case CLV_MAT_LAMBERTIAN:
       case CLV_MAT_METAL:
-        /* albedo.color(): SolidColor -> SPD, illuminant -> clamp(table*intensity) */
-        dm.albedo_spd = m.texture == CLV_TEX_SOLID ? spd_of(m.color) : illum_of(m.texture, m.intensity, true);
+        /* albedo.color(): SolidColor / checker cells -> SPD, illuminant -> clamp(table*intensity) */
+        dm.albedo_spd = plain_color ? spd_of(m.color) : illum_of(m.texture, m.intensity, true);
+        if (checker) use_checker();
         dm.mclass = m.kind == CLV_MAT_LAMBERTIAN ? MC_LAMBERTIAN : MC_METAL;
         break;
       case CLV_MAT_DIELECTRIC:
@@ -673,8 +723,16 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
         dm.mclass = MC_GLASS;
         break;
       case CLV_MAT_DIFFUSE_LIGHT:
-        /* emit: SolidColor -> color() (unclamped SPD), illuminant -> emit() (diffuse_light.rs:41-53) */
-        dm.emit_spd = m.texture == CLV_TEX_SOLID ? spd_of(m.color) : illum_of(m.texture, m.intensity, false);
+      case CLV_MAT_CONE_LIGHT:
+        /* emit: SolidColor / checkers -> color() (unclamped SPD), illuminant -> emit()
+         * (diffuse_light.rs:37-49, cone_light.rs:40-66) */
+        dm.emit_spd = plain_color ? spd_of(m.color) : illum_of(m.texture, m.intensity, false);
+        if (checker) use_checker();
+        if (m.kind == CLV_MAT_CONE_LIGHT) {
+          if (!std::isfinite(m.spread)) { *err = "cone light spread is not finite"; return CLV_EINVAL; }
+          dm.cone = 1u;
+          dm.cone_cos = cone_threshold(m.spread * (PI_F / 180.0f));  /* f32::to_radians, cone_light.rs:46 */
+        }
         dm.mclass = MC_LIGHT;
         break;
       default:
@@ -786,6 +844,27 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
           litems.push_back({obox, cen, (uint32_t)hs->lights.size()});
           hs->lights.push_back(L);
         }
+        break;
+      }
+      case CLV_OBJ_MOVING_SPHERE: {             /* moving_sphere.rs:55-82, time_0 = 0, time_1 = 1 */
+        if (o.priority) {
+          *err = "a MovingSphere cannot be a light-sampling target (the reference's random() is unimplemented)";
+          return CLV_EINVAL;
+        }
+        FlatPrim fp;
+        memset(&fp, 0, sizeof(fp));
+        fp.type = PT_MSPHERE;
+        F3 c0 = f3(p[0], p[1], p[2]), c1 = f3(p[3], p[4], p[5]);
+        float r = p[6];
+        F3 dc = c1 - c0;  /* center(t) = c0 + ((t - 0) / (1 - 0)) * (c1 - c0), moving_sphere.rs:85-90 */
+        fp.d[0] = c0.x; fp.d[1] = c0.y; fp.d[2] = c0.z; fp.d[3] = r;
+        fp.d[4] = dc.x; fp.d[5] = dc.y; fp.d[6] = dc.z;
+        Box b0 = box_from(c0 - f3(r, r, r), c0 + f3(r, r, r)), b1 = box_from(c1 - f3(r, r, r), c1 + f3(r, r, r));
+        for (int a = 0; a < 3; a++) {  /* AABB::combine; the centre at any t in [0, 1] lies between c0 and c1 */
+          obox.lo[a] = fminf(b0.lo[a], b1.lo[a]);
+          obox.hi[a] = fmaxf(b0.hi[a], b1.hi[a]);
+        }
+        local.push_back(fp);
         break;
       }
       case CLV_OBJ_BOXY: {                      /* boxy.rs:43-83 */
@@ -912,12 +991,13 @@ int rebase_host_scene(HostScene *hs, const ArenaBase &b, std::string *err) {
     info += b.info;
     memcpy(rec + 4 * u + 3, &hdr, 4);
     memcpy(rec + 4 * (u + 1) + 3, &info, 4);
-    u += type == PT_SPHERE ? PREC_SPHERE : (type == PT_TRI ? PREC_TRI : PREC_QUAD);
+    u += PREC_OF(type);
   }
   for (PrimInfo &pi : hs->prim_info) pi.material += b.mats;
   for (DevMaterial &m : hs->materials) {
     if (m.albedo_spd >= 0) m.albedo_spd += (int32_t)b.spd;
     if (m.emit_spd >= 0) m.emit_spd += (int32_t)b.spd;
+    if (m.odd_spd >= 0) m.odd_spd += (int32_t)b.spd;
   }
   if (hs->background_spd >= 0) hs->background_spd += (int32_t)b.spd;
   for (DevLightNode &n : hs->light_nodes)

@@ -34,6 +34,9 @@ struct HostScene {
  * clv_status; *err holds the detail on failure. */
 int build_host_scene(const clv_scene_desc *desc, HostScene *out, std::string *err);
 
+/* the f32 threshold below which a ConeLight of this spread (radians) is dark */
+float cone_threshold(float spread_rad);
+
 /* Offsets of a scene inside the context's device arena, in elements of each
  * array (prims and spd in float4 units / floats). */
 struct ArenaBase {

@@ -41,7 +41,7 @@ struct DevScene {
   const float *lweights;
   const DevJob *jobs;
   uint32_t has_xforms;    /* some scene of the batch has instance chains */
-  uint32_t pad;
+  uint32_t has_ext;       /* some scene of the batch has checker textures or cone lights */
 };
 
 /* structure-of-arrays path state, one slot per in-flight sample */

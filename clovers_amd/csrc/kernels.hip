@@ -219,13 +219,11 @@ __device__ __forceinline__ void xform_ray(const Xform &x, V &o, V &d) {
 }
 
 /* test one primitive record; on a hit inside [tmin, tbest] update tbest */
-__device__ __forceinline__ uint32_t prim_units(uint32_t type) {
-  return type == PT_SPHERE ? PREC_SPHERE : (type == PT_TRI ? PREC_TRI : PREC_QUAD);
-}
+__device__ __forceinline__ uint32_t prim_units(uint32_t type) { return PREC_OF(type); }
 
 template <bool HAS_XFORM>
 __device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec, uint32_t hdr, V o, V d, const WRay &wr,
-                                          float tmin, float tmax, float &t) {
+                                          float time, float tmin, float tmax, float &t) {
   uint32_t type = hdr & 3u;
   uint32_t xf = hdr >> 2;
   WRay w2 = wr;
@@ -238,6 +236,10 @@ __device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec,
     return sphere_t(xyz(f0), f1.x, o, d, tmin, tmax, t);
   }
   if (type == PT_TRI) return tri_t(rec, o, d, w2, tmin, tmax, t);
+  if (type == PT_MSPHERE) { /* MovingSphere::hit (moving_sphere.rs:106-160): the sphere at the ray's time */
+    float4 f0 = rec[0], f1 = rec[1], f2 = rec[2];
+    return sphere_t(xyz(f0) + time * xyz(f2), f1.x, o, d, tmin, tmax, t);
+  }
   return quad_t(rec, o, d, tmin, tmax, t);
 }
 
@@ -256,6 +258,7 @@ struct Lane {
   V o, d, inv, oi;
   WRay wr;
   float tb;          /* closest accepted distance so far (the reference starts at f32::MAX, trace.rs:44) */
+  float time;        /* the ray's instant in [0, 1): moving spheres are tested where they are then */
   uint32_t best;     /* record offset of the closest hit, MISS_WORD if none */
   uint32_t word;     /* node / leaf word being processed, CHILD_EMPTY when the traversal is over */
   uint32_t path;
@@ -267,6 +270,7 @@ __device__ __forceinline__ void lane_start(Lane &L, const PathBuf &pb, uint32_t 
   float4 o4 = pb.ray_o[path], d4 = pb.ray_d[path];
   L.o = xyz(o4);
   L.d = xyz(d4);
+  L.time = d4.w;
   V dd = mk(fabsf(L.d.x) < 1e-20f ? copysignf(1e-20f, L.d.x) : L.d.x,
             fabsf(L.d.y) < 1e-20f ? copysignf(1e-20f, L.d.y) : L.d.y,
             fabsf(L.d.z) < 1e-20f ? copysignf(1e-20f, L.d.z) : L.d.z);
@@ -346,8 +350,8 @@ __device__ __forceinline__ uint32_t lane_leaf(Lane &L, const DevScene &sc, bool 
     const float4 *rec = sc.prims + off;
     uint32_t hdr = as_u(rec[0].w);
     float t;
-    bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t)
-                       : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t);
+    bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t)
+                       : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t);
     if (h) {
       L.tb = t;
       L.best = off;
@@ -411,7 +415,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
       ly = rnd_pm1(s);
       if (lx * lx + ly * ly <= 1.0f) break;
     }
-    (void)rnd(s); /* time: unused by the camera, drawn to keep the stream order */
+    const float time = rnd(s); /* the ray's instant (sampler/random.rs:23); carried in ray_d.w */
     uint32_t wl = (uint32_t)(((uint64_t)pcg_next(s) * 400ull) >> 32);
     float u = ((float)x + ox) / (float)W;
     float v = ((float)y + oy) / (float)H;
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
     dir = unit(dir);
     org = org + off;
     pb.ray_o[i] = make_float4(org.x, org.y, org.z, __uint_as_float(jb.root));
-    pb.ray_d[i] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    pb.ray_d[i] = make_float4(dir.x, dir.y, dir.z, time);
     pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     pb.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pb.rng[i] = s;
@@ -504,8 +508,8 @@ __device__ __forceinline__ void lprim_at(Lane &L, const DevScene &sc, const floa
                                          uint32_t &units) {
   uint32_t hdr = as_u(rec[0].w);
   float t;
-  bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t)
-                     : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, EPS_ACNE, L.tb, t);
+  bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t)
+                     : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t);
   if (h) {
     L.tb = t;
     L.best = off;
@@ -750,18 +754,30 @@ struct Hit {
   V p, n;       /* position, normal oriented against the ray */
   bool front;
   uint32_t material;
+  float u, v;   /* surface coordinates (only for surface-checker materials) */
 };
+
+/* Sphere::get_uv (sphere.rs:58-65), MovingSphere::get_uv (moving_sphere.rs:93-101) */
+__device__ __forceinline__ void sphere_uv(V p, V c, float r, float &u, float &v) {
+  V t = (p - c) / r;
+  float phi = atan2f(t.z, t.x);
+  float theta = asinf(t.y);
+  u = 1.0f - (phi + PI_F) / (2.0f * PI_F);
+  v = (theta + PI_F / 2.0f) / PI_F;
+}
 
 /* Reconstruct the reference's HitRecord for the hit primitive: run the wrapper
  * chain forward, evaluate the primitive's record, then unwind RotateY
  * (rotate.rs:115-158: rotate back, set_face_normal with the rotated ray) and
  * Translate (translate.rs:68-77: shift back, set_face_normal with the moved ray). */
-__device__ Hit hit_record(const DevScene &sc, uint32_t off, V o, V d, float t) {
+template <bool UV>
+__device__ Hit hit_record(const DevScene &sc, uint32_t off, V o, V d, float time, float t) {
   const float4 *rec = sc.prims + off;
   float4 f0 = rec[0], f1 = rec[1];
   uint32_t hdr = as_u(f0.w), type = hdr & 3u, xf = hdr >> 2;
   Hit h;
   h.material = sc.info[as_u(f1.w)].material;
+  h.u = h.v = 0.0f;
   V dirs[4];
   uint32_t nops = 0;
   const Xform *x = sc.xforms;
@@ -785,11 +801,28 @@ __device__ Hit hit_record(const DevScene &sc, uint32_t off, V o, V d, float t) {
   }
   V p = o + t * d;
   V outward;
-  if (type == PT_SPHERE) {
-    V c = xyz(f0);
+  const bool want_uv = UV && sc.mats[h.material].tex == TX_SURFACE;
+  if (type == PT_SPHERE || type == PT_MSPHERE) {
+    V c = type == PT_SPHERE ? xyz(f0) : xyz(f0) + time * xyz(rec[2]);
     outward = unit((p - c) / f1.x);
+    if (want_uv) sphere_uv(p, c, f1.x, h.u, h.v);
   } else {
     outward = xyz(rec[3]);
+    if (want_uv) {
+      /* (u, v) = the plane coordinates (alpha, beta) of quad.rs:108-128 / triangle.rs:143-160;
+       * a triangle's edge vectors are rebuilt from its stored corners */
+      V q, eu, ev, w;
+      if (type == PT_QUAD) {
+        q = xyz(f0); eu = xyz(f1); ev = xyz(rec[2]); w = xyz(rec[4]);
+      } else {
+        q = xyz(f0); eu = xyz(f1) - q; ev = xyz(rec[2]) - q;
+        V nn = cross(eu, ev);
+        w = nn / dot(nn, nn);
+      }
+      V ph = p - q;
+      h.u = dot(w, cross(ph, ev));
+      h.v = dot(w, cross(eu, ph));
+    }
   }
   bool front = dot(d, outward) < 0.0f;
   V n = front ? outward : -outward;
@@ -940,6 +973,17 @@ __device__ float tree_pdf(const DevScene &sc, const DevJob &jb, V o, V d) {
   return p;
 }
 
+/* the SPD a hit reads from a material's texture: the plain table, or a
+ * checker's even/odd cell table (spatial_checker.rs:74-88 by world position,
+ * surface_checker.rs:72-84 by surface coordinates) */
+__device__ __forceinline__ int32_t tex_spd(const DevMaterial &m, int32_t even, const Hit &h) {
+  if (m.tex == TX_PLAIN) return even;
+  const float k = m.density_pi;
+  const float sines = m.tex == TX_SPATIAL ? sinf(k * h.p.x) * sinf(k * h.p.y) * sinf(k * h.p.z)
+                                          : sinf(k * h.u) * sinf(k * h.v);
+  return sines < 0.0f ? m.odd_spd : even;
+}
+
 __device__ __forceinline__ void rotate_wl(uint32_t hero_idx, int wl[4]) {
 #pragma unroll
   for (int j = 0; j < 4; j++) wl[j] = CLV_LAMBDA_MIN + (int)((hero_idx + 100u * (uint32_t)j) % 400u);
@@ -958,8 +1002,10 @@ __device__ __forceinline__ void gd_credit(const PathBuf &pb, const GuideArgs &ga
   }
 }
 
-/* One shading step of one path; returns true if the path continues. */
-template <uint32_t CLS, bool GUIDE>
+/* One shading step of one path; returns true if the path continues.  EXT
+ * adds what only some scenes need (checker textures, surface coordinates,
+ * cone lights); batches without them use the leaner instantiation. */
+template <uint32_t CLS, bool GUIDE, bool EXT>
 __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint32_t i, const GuideArgs &ga) {
   uint32_t meta = pb.meta[i];
   uint32_t hero_idx = meta & META_HERO_MASK;
@@ -982,11 +1028,16 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
   float4 o4 = pb.ray_o[i], d4 = pb.ray_d[i];
   V o = xyz(o4), d = xyz(d4);
   float t = pb.hit_t[i];
-  Hit h = hit_record(sc, pb.hit_p[i], o, d, t);
+  Hit h = hit_record<EXT>(sc, pb.hit_p[i], o, d, d4.w, t);
   const DevMaterial &m = sc.mats[h.material];
-  if (CLS == MC_LIGHT) { /* DiffuseLight: emit on the front face, never scatters */
-    if (h.front) {
-      const float *e = sc.spd + m.emit_spd;
+  if (CLS == MC_LIGHT) { /* DiffuseLight / ConeLight: emit on the front face (inside the cone), never scatter */
+    bool lit = h.front;
+    if (EXT && lit && m.cone) { /* cone_light.rs:46-53 */
+      const float c = -dot(d, h.n) / (sqrtf(dot(d, d)) * sqrtf(dot(h.n, h.n)));
+      lit = !(c >= -1.0f && c <= 1.0f && c < m.cone_cos);
+    }
+    if (lit) {
+      const float *e = sc.spd + (EXT ? tex_spd(m, m.emit_spd, h) : m.emit_spd);
       float4 L = pb.rad[i];
       L.x += T.x * e[wl[0] - CLV_LAMBDA_MIN];
       L.y += T.y * e[wl[1] - CLV_LAMBDA_MIN];
@@ -1002,7 +1053,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
   const bool wdep = m.thin_film || m.kind == CLV_MAT_DISPERSIVE;
   float att[4];
   {
-    const float *alb = m.albedo_spd >= 0 ? sc.spd + m.albedo_spd : nullptr;
+    const float *alb = m.albedo_spd >= 0 ? sc.spd + (EXT ? tex_spd(m, m.albedo_spd, h) : m.albedo_spd) : nullptr;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       if (wdep && j > 0) { att[j] = 0.0f; continue; }
@@ -1104,12 +1155,12 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
   if (depth + 1u > jb.max_depth) return false; /* trace() at depth > max_depth returns 0 */
   pb.thr[i] = T;
   pb.ray_o[i] = make_float4(h.p.x, h.p.y, h.p.z, o4.w); /* .w: root node of the path's scene */
-  pb.ray_d[i] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+  pb.ray_d[i] = make_float4(nd.x, nd.y, nd.z, d4.w); /* .w: the ray's instant stays with the path */
   pb.meta[i] = (meta & ~(META_DEPTH_MASK << META_DEPTH_SHIFT)) | ((depth + 1u) << META_DEPTH_SHIFT);
   return true;
 }
 
-template <bool GUIDE>
+template <bool GUIDE, bool EXT>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQueues sq, uint32_t *qout,
                                                 uint32_t *counts, uint32_t out_slot, GuideArgs ga) {
   __shared__ uint32_t agg[1 * 4 + 1];
@@ -1138,11 +1189,11 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
       if (idx < c[cls]) {
         path = sq.q[cls][idx];
         switch (cls) {
-          case MC_MISS: cont = shade_path<MC_MISS, GUIDE>(sc, pb, path, ga); break;
-          case MC_LIGHT: cont = shade_path<MC_LIGHT, GUIDE>(sc, pb, path, ga); break;
-          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN, GUIDE>(sc, pb, path, ga); break;
-          case MC_METAL: cont = shade_path<MC_METAL, GUIDE>(sc, pb, path, ga); break;
-          default: cont = shade_path<MC_GLASS, GUIDE>(sc, pb, path, ga); break;
+          case MC_MISS: cont = shade_path<MC_MISS, GUIDE, EXT>(sc, pb, path, ga); break;
+          case MC_LIGHT: cont = shade_path<MC_LIGHT, GUIDE, EXT>(sc, pb, path, ga); break;
+          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN, GUIDE, EXT>(sc, pb, path, ga); break;
+          case MC_METAL: cont = shade_path<MC_METAL, GUIDE, EXT>(sc, pb, path, ga); break;
+          default: cont = shade_path<MC_GLASS, GUIDE, EXT>(sc, pb, path, ga); break;
         }
       }
     }
@@ -1190,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, Gu
       if (jb.guiding) {
         path = pth;
         const float4 o4 = pb.ray_o[pth], d4 = pb.ray_d[pth];
-        const Hit h = hit_record(sc, pb.hit_p[pth], xyz(o4), xyz(d4), pb.hit_t[pth]);
+        const Hit h = hit_record<false>(sc, pb.hit_p[pth], xyz(o4), xyz(d4), d4.w, pb.hit_t[pth]);
         in[0] = 2.0f * (h.p.x - jb.lo[0]) / (jb.hi[0] - jb.lo[0]) - 1.0f;
         in[1] = 2.0f * (h.p.y - jb.lo[1]) / (jb.hi[1] - jb.lo[1]) - 1.0f;
         in[2] = 2.0f * (h.p.z - jb.lo[2]) / (jb.hi[2] - jb.lo[2]) - 1.0f;
@@ -1420,8 +1471,14 @@ void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
                   uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga) {
   uint32_t g = grid_for(max_paths + 64u * MC_COUNT, 256, grid_cap);
-  if (ga.on) hipLaunchKernelGGL(k_shade<true>, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
-  else hipLaunchKernelGGL(k_shade<false>, dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+  if (ga.on && sc.has_ext)
+    hipLaunchKernelGGL((k_shade<true, true>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+  else if (ga.on)
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+  else if (sc.has_ext)
+    hipLaunchKernelGGL((k_shade<false, true>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+  else
+    hipLaunchKernelGGL((k_shade<false, false>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
 }
 void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GuideArgs &ga,
                         const uint32_t *queue, const uint32_t *count, uint32_t max_paths) {

@@ -10,11 +10,11 @@ internally tagged ``kind`` for objects, materials and textures
 (a shared-material name or an inline material, materials.rs:33-39) and
 untagged ColorInit (colorinit.rs:17-43).
 
-Out of scope (raise NotImplementedError): mesh files (STL/PLY/GLTF), moving
-spheres, constant media, cone lights, isotropic, checker textures, and the
-illuminants other than D65/D50.  A list nested inside RotateY/Translate is
-rejected too: the reference's HitableList::hit returns None as soon as any
-child misses (hitable.rs:158), which no scene relies on.
+Out of scope (raise NotImplementedError): GLTF files, constant media,
+isotropic, and the illuminants other than D65/D50.  A list nested inside
+RotateY/Translate is rejected too: the reference's HitableList::hit returns
+None as soon as any child misses (hitable.rs:158), which no scene relies on.
+STL and PLY meshes are read by clovers_amd.meshfile.
 """
 import json
 
@@ -22,6 +22,7 @@ import numpy as np
 
 from . import abi
 from .color import color_init_to_xyz_e, srgb_to_xyz_e
+from .meshfile import mesh_triangles
 
 
 class SceneDescription:
@@ -130,17 +131,31 @@ class SceneDescription:
 
 # ----------------------------------------------------------------- materials
 def _texture(t, default_xyz):
-    """Texture JSON -> (texture kind, Xyz<E> colour, intensity). textures.rs:22-109."""
+    """Texture JSON -> MATERIAL_DTYPE fields (texture kind, Xyz<E> colours,
+    intensity, checker density). textures.rs:22-109."""
     if t is None:  # Texture::default() = SolidColor::default() = Xyz<E>(0.5,0.5,0.5) (solid_color.rs:50-56)
-        return abi.TEX_SOLID, np.array(default_xyz, np.float32), 1.0
+        return dict(texture=abi.TEX_SOLID, color=np.array(default_xyz, np.float32))
     kind = t.get("kind")
     if kind == "SolidColor":
-        return abi.TEX_SOLID, color_init_to_xyz_e(t["color"]), 1.0
+        return dict(texture=abi.TEX_SOLID, color=color_init_to_xyz_e(t["color"]))
     if kind == "IlluminantD65":
-        return abi.TEX_ILLUMINANT_D65, np.zeros(3, np.float32), float(t.get("intensity", 1.0))
+        return dict(texture=abi.TEX_ILLUMINANT_D65, color=np.zeros(3, np.float32), intensity=float(t.get("intensity", 1.0)))
     if kind == "IlluminantD50":
-        return abi.TEX_ILLUMINANT_D50, np.zeros(3, np.float32), float(t.get("intensity", 1.0))
+        return dict(texture=abi.TEX_ILLUMINANT_D50, color=np.zeros(3, np.float32), intensity=float(t.get("intensity", 1.0)))
+    if kind in ("SpatialChecker", "SurfaceChecker"):
+        # defaults: even Xyz<E>(0.8, 0.8, 0.8), odd Xyz<E>(0.5, 0.5, 0.5), density 1 (spatial) / 10 (surface)
+        # (spatial_checker.rs:46-61, surface_checker.rs:44-59)
+        even = color_init_to_xyz_e(t["even"]) if "even" in t else np.full(3, 0.8, np.float32)
+        odd = color_init_to_xyz_e(t["odd"]) if "odd" in t else np.full(3, 0.5, np.float32)
+        spatial = kind == "SpatialChecker"
+        return dict(texture=abi.TEX_SPATIAL_CHECKER if spatial else abi.TEX_SURFACE_CHECKER, color=even,
+                    odd_color=odd, density=float(t.get("density", 1.0 if spatial else 10.0)))
     raise NotImplementedError("texture kind %r is not supported" % kind)
+
+
+def _set(r, fields):
+    for k, v in fields.items():
+        r[k] = v
 
 
 def material_record(m):
@@ -156,12 +171,10 @@ def material_record(m):
     kind = m.get("kind", "Lambertian") if m else "Lambertian"
     if kind == "Lambertian":
         r["kind"] = abi.MAT_LAMBERTIAN
-        tk, col, inten = _texture(m.get("albedo") if m else None, (0.5, 0.5, 0.5))
-        r["texture"], r["color"], r["intensity"] = tk, col, inten
+        _set(r, _texture(m.get("albedo") if m else None, (0.5, 0.5, 0.5)))
     elif kind == "Metal":
         r["kind"] = abi.MAT_METAL
-        tk, col, inten = _texture(m.get("albedo"), (0.5, 0.5, 0.5))
-        r["texture"], r["color"], r["intensity"] = tk, col, inten
+        _set(r, _texture(m.get("albedo"), (0.5, 0.5, 0.5)))
         r["fuzz"] = float(m.get("fuzz", 0.0))   # not clamped on the serde path (metal.rs:60)
     elif kind == "Dielectric":
         r["kind"] = abi.MAT_DIELECTRIC
@@ -175,8 +188,14 @@ def material_record(m):
     elif kind == "DiffuseLight":
         r["kind"] = abi.MAT_DIFFUSE_LIGHT
         # DiffuseLight::default(): SolidColor Xyz(100,100,100) (diffuse_light.rs:23-30)
-        tk, col, inten = _texture(m.get("emit"), (100.0, 100.0, 100.0))
-        r["texture"], r["color"], r["intensity"] = tk, col, inten
+        _set(r, _texture(m.get("emit"), (100.0, 100.0, 100.0)))
+    elif kind == "ConeLight":
+        # both fields are required by the scene-file schema (cone_light.rs:13-18: no serde defaults)
+        if "spread" not in m or "emit" not in m:
+            raise ValueError("ConeLight needs 'spread' and 'emit'")
+        r["kind"] = abi.MAT_CONE_LIGHT
+        r["spread"] = float(m["spread"])
+        _set(r, _texture(m["emit"], (100.0, 100.0, 100.0)))
     else:
         raise NotImplementedError("material kind %r is not supported" % kind)
     tf = m.get("thin_film") if m else None
@@ -188,7 +207,7 @@ def material_record(m):
 
 
 # ------------------------------------------------------------------- objects
-_PRIM_KINDS = ("Sphere", "Quad", "Triangle", "Boxy")
+_PRIM_KINDS = ("Sphere", "Quad", "Triangle", "Boxy", "MovingSphere")
 
 
 def _leaf_params(o):
@@ -201,13 +220,16 @@ def _leaf_params(o):
         return abi.OBJ_TRIANGLE_QUV, list(o["q"]) + list(o["u"]) + list(o["v"])
     if k == "Boxy":
         return abi.OBJ_BOXY, list(o["corner_0"]) + list(o["corner_1"])
+    if k == "MovingSphere":  # time_0/time_1 of the file are not read: the object moves over [0, 1] (objects.rs:104-109)
+        return abi.OBJ_MOVING_SPHERE, list(o["center_0"]) + list(o["center_1"]) + [o["radius"]]
     raise NotImplementedError("object kind %r is not supported" % k)
 
 
 class _Loader:
-    def __init__(self, scene, shared):
+    def __init__(self, scene, shared, mesh_root=None):
         self.scene = scene
         self.shared = shared   # list of (name, material index); the default (name "") last
+        self.mesh_root = mesh_root
 
     def material(self, init):
         """MaterialInit -> material index (objects.rs:159-188)."""
@@ -230,22 +252,41 @@ class _Loader:
             return self.wrapped(o["object"], ops + [(abi.XF_TRANSLATE, list(o["offset"]))])
         if k == "RotateY":
             return self.wrapped(o["object"], ops + [(abi.XF_ROTATE_Y, [float(o["angle"]), 0.0, 0.0])])
-        if k == "ObjectList":
+        if k in ("ObjectList", "STL", "PLY"):
             if ops:
                 raise NotImplementedError("a list inside RotateY/Translate (hitable.rs:158 returns None on any miss)")
-            return [("list", o)]
+            return [("list" if k == "ObjectList" else "mesh", o)]
         if k not in _PRIM_KINDS:
             raise NotImplementedError("object kind %r is not supported" % k)
         return [("prim", o, ops)]
+
+    def mesh_material(self, init):
+        """STL/PLY material: a shared name must exist (stl.rs:50, ply.rs:43
+        unwrap the lookup: no fallback to the default); the default
+        MaterialInit is the name "" of the appended default material."""
+        if init is None:
+            init = ""
+        if isinstance(init, str):
+            for name, idx in self.shared:
+                if name == init:
+                    return idx
+            raise ValueError("mesh material %r not found (the reference panics)" % init)
+        return self.scene.add_material(material_record(init))
 
     def add(self, o, priority):
         for entry in self.wrapped(o, []):
             if entry[0] == "list":
                 # top-level list: flatten one level (scenefile.rs:84-94)
                 for child in entry[1]["objects"]:
-                    if child["kind"] == "ObjectList":
+                    if child["kind"] in ("ObjectList", "STL", "PLY"):
                         raise ValueError("Recursing too hard in scene initialization")
                     self.add(child, priority)
+                continue
+            if entry[0] == "mesh":
+                # one Triangle::from_coordinates per face, flattened (objects.rs:133-141)
+                tris = mesh_triangles(entry[1], self.mesh_root)
+                mat = self.mesh_material(entry[1].get("material"))
+                self.scene.add_objects(abi.OBJ_TRIANGLE, tris.reshape(-1, 9), mat, priority)
                 continue
             _, prim, ops = entry
             kind, params = _leaf_params(prim)
@@ -254,9 +295,11 @@ class _Loader:
             self.scene.add_object(kind, params, mat, priority, xf)
 
 
-def load_scene(src, bvh_algorithm=abi.BVH_SAH):
+def load_scene(src, bvh_algorithm=abi.BVH_SAH, mesh_root=None):
     """Scene file (path, JSON text or parsed dict) -> SceneDescription.
-    Mirrors SceneFile::initialize (scenefile.rs:34-117)."""
+    Mirrors SceneFile::initialize (scenefile.rs:34-117).  Relative STL/PLY
+    paths are read from `mesh_root` (default: the working directory, as the
+    reference does)."""
     if isinstance(src, dict):
         sf = src
     elif isinstance(src, str) and src.lstrip().startswith("{"):
@@ -277,7 +320,7 @@ def load_scene(src, bvh_algorithm=abi.BVH_SAH):
     for m in sf.get("materials", []):
         shared.append((m.get("name", ""), s.add_material(material_record(m))))
     shared.append(("", s.add_material(material_record(None))))   # SharedMaterial::default() (scenefile.rs:58-59)
-    ld = _Loader(s, shared)
+    ld = _Loader(s, shared, mesh_root)
     for o in sf.get("objects", []):
         ld.add(o, bool(o.get("priority", False)))
     return s

@@ -12,10 +12,12 @@ flattens mesh lists into one hitable per triangle (scenefile.rs:84-94).
                Reference analogue: scenes/cornell.json.
   spheres      22x22 grid of r=0.2 spheres (80% Lambertian albedo U*U, 15% metal
                fuzz U[0,0.5], 5% glass n=1.5) + 3 r=1 spheres + ground r=1000.
-               Reference analogue: scenes/one_weekend.js (static spheres).
+               Reference analogue: scenes/one_weekend.json (static spheres; with
+               moving=True the Lambertian ones rise as MovingSpheres, as there).
   prism        Cauchy-glass triangular prism (12 triangles), glass sphere r=0.5,
                grey Lambertian floor, two 0.1x0.1 emitters of radiance 500.
                Reference analogue: scenes/dispersive.json.
+  features     checker textures, cone light and moving spheres in one room.
   mesh         Frequency-224 geodesic icosphere (1,003,520 triangles) displaced
                by 4-octave value noise (amplitude 0.15), thin-film dielectric,
                quad light, 4 camera views.
@@ -71,9 +73,13 @@ def cornell():
 
 
 # ----------------------------------------------------------------- spheres
-def spheres_json(seed=1):
+def spheres_json(seed=1, moving=False):
+    """moving=True: the Lambertian small spheres rise by U[0, 0.5] over the
+    shutter as MovingSpheres, as in the reference's scenes/one_weekend.json;
+    the rises come from a second stream, so the layout is the static one."""
     r = _rng(seed)
     U = lambda: float(r.random())
+    rise = _rng(seed + 7919)
     mats = [
         {"name": "ground material", "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": [0.5, 0.5, 0.5]}},
         {"name": "glass", "kind": "Dielectric", "refractive_index": 1.5},
@@ -96,7 +102,12 @@ def spheres_json(seed=1):
             if choose < 0.8:
                 col = [U() * U(), U() * U(), U() * U()]
                 mats.append({"name": name, "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": col}})
-                objs.append({"kind": "Sphere", "center": center, "radius": 0.2, "material": name})
+                if moving:
+                    c1 = [center[0], center[1] + 0.5 * float(rise.random()), center[2]]
+                    objs.append({"kind": "MovingSphere", "center_0": center, "center_1": c1, "radius": 0.2,
+                                 "material": name})
+                else:
+                    objs.append({"kind": "Sphere", "center": center, "radius": 0.2, "material": name})
             elif choose < 0.95:
                 col = [0.5 + 0.5 * U(), 0.5 + 0.5 * U(), 0.5 + 0.5 * U()]
                 fuzz = 0.5 * U()
@@ -115,8 +126,53 @@ def spheres_json(seed=1):
     }
 
 
-def spheres(seed=1):
-    return load_scene(spheres_json(seed))
+def spheres(seed=1, moving=False):
+    return load_scene(spheres_json(seed, moving))
+
+
+# -------------------------------------------------------------- features
+def features_json():
+    """Every texture, light and motion kind of the scene schema in one box:
+    a spatial-checker floor, a surface-checker back wall and triangle, moving
+    spheres (one inside Translate), a surface-checker metal sphere, a
+    spatial-checker box under Translate{RotateY}, a ceiling lamp and a cone
+    light aimed into the room (both light-sampling targets)."""
+    chk = lambda kind, e, o, d: {"kind": kind, "even": {"xyz_e": e}, "odd": {"xyz_e": o}, "density": d}
+    mats = [
+        {"name": "floor", "kind": "Lambertian", "albedo": chk("SpatialChecker", [0.8, 0.8, 0.8], [0.2, 0.25, 0.3], 1.5)},
+        {"name": "wall", "kind": "Lambertian", "albedo": chk("SurfaceChecker", [0.7, 0.3, 0.2], [0.3, 0.6, 0.3], 12.0)},
+        {"name": "ball", "kind": "Metal", "fuzz": 0.2, "albedo": chk("SurfaceChecker", [0.9, 0.9, 0.9], [0.4, 0.3, 0.8], 10.0)},
+        {"name": "blue", "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": [0.2, 0.3, 0.8]}},
+        {"name": "boxm", "kind": "Metal", "fuzz": 0.05, "albedo": chk("SpatialChecker", [0.9, 0.8, 0.5], [0.3, 0.3, 0.3], 4.0)},
+        {"name": "lamp", "kind": "DiffuseLight", "emit": {"kind": "IlluminantD65", "intensity": 0.12}},
+        {"name": "cone", "kind": "ConeLight", "spread": 25.0, "emit": chk("SurfaceChecker", [40, 40, 40], [5, 5, 5], 3.0)},
+        {"name": "glass", "kind": "Dielectric", "refractive_index": 1.5},
+    ]
+    objs = [
+        {"kind": "Quad", "q": [-4, 0, -4], "u": [8, 0, 0], "v": [0, 0, 8], "material": "floor"},
+        {"kind": "Quad", "q": [-4, 0, -4], "u": [0, 4, 0], "v": [8, 0, 0], "material": "wall"},
+        {"kind": "Quad", "priority": True, "q": [-1, 3.99, -1], "u": [2, 0, 0], "v": [0, 0, 2], "material": "lamp"},
+        {"kind": "Quad", "priority": True, "q": [-3.9, 0.5, 2.5], "u": [0, 0, -1], "v": [0, 1, 0], "material": "cone"},
+        {"kind": "Triangle", "q": [1.5, 0.01, -3.5], "u": [2, 0, 0], "v": [0, 2.5, 0], "material": "wall"},
+        {"kind": "MovingSphere", "center_0": [-1.5, 0.5, 0.5], "center_1": [-1.0, 1.2, 0.5], "radius": 0.5, "material": "blue"},
+        {"kind": "Translate", "offset": [1.6, 0, 1.0], "object":
+            {"kind": "MovingSphere", "center_0": [0, 0.4, 0], "center_1": [0.6, 0.4, 0], "radius": 0.4, "material": "glass"}},
+        {"kind": "Sphere", "center": [0.2, 0.8, -1.2], "radius": 0.8, "material": "ball"},
+        {"kind": "Translate", "offset": [-2.3, 0, -2.0], "object":
+            {"kind": "RotateY", "angle": 25, "object": {"kind": "Boxy", "corner_0": [0, 0, 0], "corner_1": [1, 1.6, 1], "material": "boxm"}}},
+    ]
+    return {
+        "time_0": 0, "time_1": 1,
+        "camera": {"look_from": [0.0, 2.2, 7.5], "look_at": [0.0, 1.0, 0.0], "up": [0.0, 1.0, 0.0],
+                   "vertical_fov": 45.0, "aperture": 0.02, "focus_distance": 7.0},
+        "background_color": [0.05, 0.05, 0.08],
+        "objects": objs,
+        "materials": mats,
+    }
+
+
+def features():
+    return load_scene(features_json())
 
 
 # ------------------------------------------------------------------- prism

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CLV_ABI_VERSION 2u
+#define CLV_ABI_VERSION 3u
 
 typedef enum {
   CLV_OK = 0,
@@ -50,7 +50,11 @@ typedef enum {
   CLV_OBJ_QUAD = 1,     /* p = q.xyz, u.xyz, v.xyz                        (objects/quad.rs:248-270) */
   CLV_OBJ_TRIANGLE = 2, /* p = a.xyz, b.xyz, c.xyz (absolute corners)     (objects/triangle.rs:106-117) */
   CLV_OBJ_BOXY = 3,     /* p = corner_0.xyz, corner_1.xyz                  (objects/boxy.rs:43-83) */
-  CLV_OBJ_TRIANGLE_QUV = 4 /* p = q.xyz, u.xyz, v.xyz (scene-file Triangle)  (objects/triangle.rs:61-101) */
+  CLV_OBJ_TRIANGLE_QUV = 4, /* p = q.xyz, u.xyz, v.xyz (scene-file Triangle)  (objects/triangle.rs:61-101) */
+  CLV_OBJ_MOVING_SPHERE = 5 /* p = center_0.xyz, center_1.xyz, radius; time_0 = 0, time_1 = 1
+                               (objects/moving_sphere.rs, objects.rs:104-109).  Never a light-sampling
+                               target: its sampler is unimplemented in the reference (a panic), so
+                               priority = 1 is CLV_EINVAL */
 } clv_object_kind;
 
 /* One top-level hitable of the reference's flattened list (scenefile.rs:63-103):
@@ -87,20 +91,25 @@ typedef enum {
   CLV_MAT_METAL = 1,          /* materials/metal.rs       albedo texture, fuzz */
   CLV_MAT_DIELECTRIC = 2,     /* materials/dielectric.rs  refractive_index, color (Xyz<E>) */
   CLV_MAT_DISPERSIVE = 3,     /* materials/dispersive.rs  cauchy_a, cauchy_b (um^2) */
-  CLV_MAT_DIFFUSE_LIGHT = 4   /* materials/diffuse_light.rs  emit texture */
+  CLV_MAT_DIFFUSE_LIGHT = 4,  /* materials/diffuse_light.rs  emit texture */
+  CLV_MAT_CONE_LIGHT = 5      /* materials/cone_light.rs  emit texture, spread (degrees) */
 } clv_material_kind;
 
 /* Texture kinds: clovers/src/textures.rs:22-109 (in-scope subset). */
 typedef enum {
   CLV_TEX_SOLID = 0,          /* SolidColor: color = Xyz<E> (textures/solid_color.rs) */
   CLV_TEX_ILLUMINANT_D65 = 1, /* CIE D65 table x intensity (build.rs:66-112) */
-  CLV_TEX_ILLUMINANT_D50 = 2  /* CIE D50 table x intensity */
+  CLV_TEX_ILLUMINANT_D50 = 2, /* CIE D50 table x intensity */
+  CLV_TEX_SPATIAL_CHECKER = 3, /* even = color, odd = odd_color, density; by world position
+                                  (textures/spatial_checker.rs) */
+  CLV_TEX_SURFACE_CHECKER = 4  /* even = color, odd = odd_color, density; by surface (u, v)
+                                  (textures/surface_checker.rs) */
 } clv_texture_kind;
 
 typedef struct {
   uint32_t kind;          /* clv_material_kind */
-  uint32_t texture;       /* clv_texture_kind of albedo (Lambertian/Metal) or emit (DiffuseLight) */
-  float color[3];         /* SolidColor / Dielectric color, already converted to Xyz<E> */
+  uint32_t texture;       /* clv_texture_kind of albedo (Lambertian/Metal) or emit (DiffuseLight/ConeLight) */
+  float color[3];         /* SolidColor / checker even cells / Dielectric color, already converted to Xyz<E> */
   float intensity;        /* illuminant intensity (default 1.0) */
   float fuzz;             /* Metal; NOT clamped (the scene-file path does not clamp, metal.rs:60) */
   float refractive_index; /* Dielectric (default 1.5) */
@@ -109,8 +118,10 @@ typedef struct {
   uint32_t thin_film;     /* 1: ThinFilm layer present (materials.rs:63-103, thin_film.rs) */
   float film_index;       /* ThinFilm refractive_index (default 1.5) */
   float film_thickness;   /* ThinFilm thickness in nm (default 500) */
-  uint32_t _pad;
-} clv_material;           /* 56 bytes */
+  float density;          /* checker textures: cells per unit (spatial default 1, surface default 10) */
+  float odd_color[3];     /* checker textures: odd cells, Xyz<E> */
+  float spread;           /* ConeLight: half-angle of the emitting cone in degrees */
+} clv_material;           /* 72 bytes */
 
 /* CameraInit + scene times (camera.rs:40-54, scenefile.rs:21-29). */
 typedef struct {

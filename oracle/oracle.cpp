@@ -351,24 +351,40 @@ struct ScatterRecord {
 struct Material {
   int kind = CLV_MAT_LAMBERTIAN;
   int tex = CLV_TEX_SOLID;
-  std::vector<Float> spd;      /* SolidColor SPD of albedo/emit, or dielectric color SPD */
+  std::vector<Float> spd;      /* SolidColor SPD of albedo/emit, checker even cells, or dielectric color SPD */
+  std::vector<Float> spd_odd;  /* checker odd cells */
   const float *illum = nullptr; int illum_first = 0, illum_count = 0;
   Float intensity = 1.0f, fuzz = 0.0f, ior = 1.5f, ca = 1.5046f, cb = 0.0042f;
   bool thin_film = false; Float film_ior = 1.5f, film_t = 500.0f;
+  Float density = 1.0f;        /* checker textures */
+  Float spread = 10.0f;        /* ConeLight, degrees */
 
-  /* texture color / emit (textures.rs:113-121, solid_color.rs:63-67, build.rs:88-105) */
-  Float tex_color(int wl) const {
+  /* texture color (textures.rs:113-121): SolidColor (solid_color.rs:63-67),
+   * SpatialChecker by world position (spatial_checker.rs:74-88), SurfaceChecker
+   * by surface (u, v) (surface_checker.rs:72-84), illuminants clamped to
+   * [0, 1] (build.rs:88-105) */
+  Float tex_color(int wl, const HitRecord &rec) const {
     if (tex == CLV_TEX_SOLID) return spd[wl - MIN_WAVELENGTH];
+    if (tex == CLV_TEX_SPATIAL_CHECKER) {
+      Float dens = density * PI;
+      Float sines = 1.0f * sinf(dens * rec.position.x) * sinf(dens * rec.position.y) * sinf(dens * rec.position.z);
+      return sines < 0.0f ? spd_odd[wl - MIN_WAVELENGTH] : spd[wl - MIN_WAVELENGTH];
+    }
+    if (tex == CLV_TEX_SURFACE_CHECKER) {
+      Float dens = density * PI;
+      Float sines = 1.0f * sinf(dens * rec.u) * sinf(dens * rec.v);
+      return sines < 0.0f ? spd_odd[wl - MIN_WAVELENGTH] : spd[wl - MIN_WAVELENGTH];
+    }
     Float c = 0.0f;
     int i = wl - illum_first;
     if (i >= 0 && i < illum_count) c = illum[i] * intensity;
     return fminf(fmaxf(c, 0.0f), 1.0f);
   }
   Float tex_emit(int wl) const {
-    if (tex == CLV_TEX_SOLID) return 0.0f;
     int i = wl - illum_first;
     return (i >= 0 && i < illum_count) ? illum[i] * intensity : 0.0f;
   }
+  bool is_illuminant() const { return tex == CLV_TEX_ILLUMINANT_D65 || tex == CLV_TEX_ILLUMINANT_D50; }
   /* thin_film.rs:31-54 */
   Float interference(V3 direction, int wl, const HitRecord &rec) const {
     Float n1 = 1.0f, n2 = film_ior, n_ratio = n1 / n2;
@@ -386,18 +402,24 @@ struct Material {
     Float tf = thin_film ? interference(ray.direction, wl, rec) : 1.0f;
     Float k;
     switch (kind) {
-      case CLV_MAT_LAMBERTIAN: case CLV_MAT_METAL: k = tex_color(wl); break;
+      case CLV_MAT_LAMBERTIAN: case CLV_MAT_METAL: k = tex_color(wl, rec); break;
       case CLV_MAT_DIELECTRIC: k = spd[wl - MIN_WAVELENGTH]; break; /* dielectric.rs:79-81 */
       case CLV_MAT_DISPERSIVE: k = 1.0f; break;                     /* dispersive.rs:121-123 */
-      default: k = fminf(fmaxf(tex_color(wl), 0.0f), 1.0f); break;  /* diffuse_light.rs:55-57 */
+      default: k = fminf(fmaxf(tex_color(wl, rec), 0.0f), 1.0f); break;  /* diffuse_light.rs:55-57, cone_light.rs:69-71 */
     }
     return k * tf;
   }
-  /* diffuse_light.rs:41-53; other kinds emit 0 */
-  Float emit(int wl, const HitRecord &rec) const {
-    if (kind != CLV_MAT_DIFFUSE_LIGHT) return 0.0f;
+  /* diffuse_light.rs:37-49, cone_light.rs:40-66; other kinds emit 0.  Plain
+   * colours and checkers emit their color(), illuminants their table. */
+  Float emit(const Ray &ray, int wl, const HitRecord &rec) const {
+    if (kind != CLV_MAT_DIFFUSE_LIGHT && kind != CLV_MAT_CONE_LIGHT) return 0.0f;
     if (!rec.front_face) return 0.0f;
-    if (tex == CLV_TEX_SOLID) return tex_color(wl);
+    if (kind == CLV_MAT_CONE_LIGHT) {
+      Float spread_radians = spread * (PI / 180.0f);    /* f32::to_radians */
+      Float angle = acosf(-dot(ray.direction, rec.normal) / (magnitude(ray.direction) * magnitude(rec.normal)));
+      if (angle > spread_radians) return 0.0f;
+    }
+    if (!is_illuminant()) return tex_color(wl, rec);
     return tex_emit(wl);
   }
   /* dielectric.rs:35-70 / dispersive.rs:68-115 */
@@ -446,7 +468,7 @@ struct Material {
         return true;
       }
       default:
-        return false;           /* DiffuseLight never scatters */
+        return false;           /* DiffuseLight and ConeLight never scatter */
     }
   }
   /* lambertian.rs:39-46; other kinds: None (materials.rs:120-122) */
@@ -545,6 +567,54 @@ struct Sphere : Hitable {     /* objects/sphere.rs */
   bool has_random() const override { return true; }
   V3 centroid() const override { return center; }
   Hitable *clone() const override { return new Sphere(*this); }
+};
+
+/* objects/moving_sphere.rs; built with time_0 = 0, time_1 = 1 (objects.rs:104-109) */
+struct MovingSphere : Hitable {
+  V3 c0, c1; Float t0, t1, radius; const Material *mat; AABB box; uint32_t id;
+  MovingSphere(V3 a, V3 b, Float r, const Material *m, uint32_t pid) : c0(a), c1(b), t0(0.0f), t1(1.0f), radius(r), mat(m), id(pid) {
+    AABB b0 = AABB::from_coords(a - v3(r, r, r), a + v3(r, r, r));
+    AABB b1 = AABB::from_coords(b - v3(r, r, r), b + v3(r, r, r));
+    box = AABB::combine(b0, b1);
+  }
+  V3 center(Float time) const { return c0 + ((time - t0) / (t1 - t0)) * (c1 - c0); } /* :85-90 */
+  void get_uv(V3 p, Float time, Float *u, Float *v) const {                           /* :93-101 */
+    V3 t = (p - center(time)) / radius;
+    Float phi = atan2f(t.z, t.x);
+    Float theta = asinf(t.y);
+    *u = 1.0f - (phi + PI) / (2.0f * PI);
+    *v = (theta + PI / 2.0f) / PI;
+  }
+  bool make_record(const Ray &ray, Float distance, HitRecord *rec) const {
+    V3 position = ray.evaluate(distance);
+    V3 outward = normalize((position - center(ray.time)) / radius);
+    rec->distance = distance;
+    rec->position = position;
+    get_uv(position, ray.time, &rec->u, &rec->v);
+    rec->material = mat;
+    rec->prim_id = id;
+    rec->set_face_normal(ray, outward);
+    return true;
+  }
+  bool hit(const Ray &ray, Float dmin, Float dmax, HitRecord *rec) const override { /* :106-160 */
+    V3 oc = ray.origin - center(ray.time);
+    Float a = norm_squared(ray.direction);
+    Float half_b = dot(oc, ray.direction);
+    Float c = norm_squared(oc) - radius * radius;
+    Float disc = half_b * half_b - a * c;
+    if (disc > 0.0f) {
+      Float root = sqrtf(disc);
+      Float d = (-half_b - root) / a;
+      if (d < dmax && d > dmin) return make_record(ray, d, rec);
+      d = (-half_b + root) / a;
+      if (d < dmax && d > dmin) return make_record(ray, d, rec);
+    }
+    return false;
+  }
+  const AABB *aabb() const override { return &box; }
+  Float pdf_value(V3, V3, int, Float) const override { return 0.0f; } /* :167-177 */
+  V3 centroid() const override { return center(0.5f); }
+  Hitable *clone() const override { return new MovingSphere(*this); }
 };
 
 /* Quad (objects/quad.rs) and Triangle (objects/triangle.rs) share the plane test */
@@ -960,6 +1030,7 @@ static HPtr make_object(const clv_scene_desc *d, uint32_t i, const std::vector<M
     }
     case CLV_OBJ_TRIANGLE_QUV: h.reset(new Planar(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]), m, pid, true)); break;
     case CLV_OBJ_BOXY: h.reset(new Boxy(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), m, pid)); break;
+    case CLV_OBJ_MOVING_SPHERE: h.reset(new MovingSphere(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), p[6], m, pid)); break;
     default: *err = "unknown object kind"; return HPtr();
   }
   if (o.xform != CLV_NO_XFORM) {
@@ -993,7 +1064,13 @@ static Scene *scene_create(const clv_scene_desc *d, int bvh_alg, std::string *er
     m.thin_film = cm.thin_film != 0;
     m.film_ior = cm.film_index;
     m.film_t = cm.film_thickness;
-    if (m.tex == CLV_TEX_SOLID || m.kind == CLV_MAT_DIELECTRIC) m.spd = make_spd(cm.color);
+    m.density = cm.density;
+    m.spread = cm.spread;
+    if (m.tex == CLV_TEX_SOLID || m.tex == CLV_TEX_SPATIAL_CHECKER || m.tex == CLV_TEX_SURFACE_CHECKER ||
+        m.kind == CLV_MAT_DIELECTRIC)
+      m.spd = make_spd(cm.color);
+    if (m.tex == CLV_TEX_SPATIAL_CHECKER || m.tex == CLV_TEX_SURFACE_CHECKER) m.spd_odd = make_spd(cm.odd_color);
+    if (m.kind > CLV_MAT_CONE_LIGHT || m.tex > CLV_TEX_SURFACE_CHECKER) { *err = "unknown material or texture kind"; return nullptr; }
     if (m.tex == CLV_TEX_ILLUMINANT_D65) { m.illum = cie_d65; m.illum_first = CIE_D65_FIRST_NM; m.illum_count = CIE_D65_COUNT; }
     if (m.tex == CLV_TEX_ILLUMINANT_D50) { m.illum = cie_d50; m.illum_first = CIE_D50_FIRST_NM; m.illum_count = CIE_D50_COUNT; }
     s->materials.push_back(m);
@@ -1028,7 +1105,7 @@ static void trace(const Ray &ray, const Scene &scene, uint32_t depth, uint32_t m
   if (!scene.bvh_root->hit(ray, EPSILON_SHADOW_ACNE, FLT_MAX, &rec)) { for (int i = 0; i < 4; i++) out[i] = bg[i]; return; }
   const Material *m = rec.material;
   Float emitted[4];
-  for (int i = 0; i < 4; i++) emitted[i] = m->emit(wls[i], rec);
+  for (int i = 0; i < 4; i++) emitted[i] = m->emit(ray, wls[i], rec);
   ScatterRecord sr;
   if (!m->scatter(ray, rec, rng, &sr)) { for (int i = 0; i < 4; i++) out[i] = emitted[i]; return; }
   Float att[4];

@@ -91,7 +91,8 @@ def test_object_list_flattening_and_priority():
 
 def test_unsupported_inputs_raise():
     sf = synth.cornell_json()
-    sf["objects"].append({"kind": "MovingSphere", "center_0": [0, 0, 0], "center_1": [0, 1, 0], "radius": 1})
+    sf["objects"].append({"kind": "ConstantMedium", "density": 0.1,
+                          "boundary": {"kind": "Sphere", "center": [0, 0, 0], "radius": 1}})
     with pytest.raises(NotImplementedError):
         load_scene(sf)
     sf = synth.cornell_json()
@@ -99,6 +100,53 @@ def test_unsupported_inputs_raise():
                           "object": {"kind": "ObjectList", "objects": []}})
     with pytest.raises(NotImplementedError):
         load_scene(sf)
+
+
+def test_moving_sphere_checkers_and_cone_light():
+    """MovingSphere (objects.rs:104-109: the file's times are not read), the
+    two checker textures with their serde defaults (spatial_checker.rs:46-61,
+    surface_checker.rs:44-59) and ConeLight (cone_light.rs: both fields required)."""
+    sf = synth.cornell_json()
+    sf["materials"] += [
+        {"name": "sc", "kind": "Lambertian", "albedo": {"kind": "SpatialChecker"}},
+        {"name": "uc", "kind": "Metal", "albedo": {"kind": "SurfaceChecker", "even": [1, 0, 0], "odd": {"xyz_e": [0.1, 0.2, 0.3]}}},
+        {"name": "cone", "kind": "ConeLight", "spread": 12.5, "emit": {"kind": "SurfaceChecker", "density": 3}},
+        {"name": "cone65", "kind": "ConeLight", "spread": 30, "emit": {"kind": "IlluminantD65", "intensity": 2}},
+    ]
+    sf["objects"] += [
+        {"kind": "MovingSphere", "center_0": [1, 2, 3], "center_1": [4, 5, 6], "time_0": 0.3, "time_1": 0.7,
+         "radius": 0.5, "material": "sc"},
+        {"kind": "Quad", "q": [0, 0, 0], "u": [1, 0, 0], "v": [0, 1, 0], "material": "cone"},
+    ]
+    s = load_scene(sf)
+    o = s.objects
+    assert o["kind"][-2] == abi.OBJ_MOVING_SPHERE
+    assert list(o["p"][-2][:7]) == [1, 2, 3, 4, 5, 6, 0.5]
+    mats = {k: s.materials[o["material"][i]] for k, i in (("sc", -2), ("cone", -1))}
+    sc = mats["sc"]
+    assert sc["texture"] == abi.TEX_SPATIAL_CHECKER and sc["density"] == 1.0
+    assert np.allclose(sc["color"], 0.8) and np.allclose(sc["odd_color"], 0.5)
+    cone = mats["cone"]
+    assert cone["kind"] == abi.MAT_CONE_LIGHT and cone["spread"] == np.float32(12.5)
+    assert cone["texture"] == abi.TEX_SURFACE_CHECKER and cone["density"] == 3.0
+    by_kind = {(int(m["kind"]), int(m["texture"])): m for m in s.materials}
+    uc = by_kind[(abi.MAT_METAL, abi.TEX_SURFACE_CHECKER)]
+    assert uc["density"] == 10.0 and np.allclose(uc["odd_color"], [0.1, 0.2, 0.3])
+    assert np.allclose(uc["color"], srgb_to_xyz_e([1, 0, 0]), atol=1e-6)
+    c65 = by_kind[(abi.MAT_CONE_LIGHT, abi.TEX_ILLUMINANT_D65)]
+    assert c65["intensity"] == 2.0 and c65["spread"] == 30.0
+    clovers_amd.validate(s)
+    with pytest.raises(ValueError):
+        load_scene({**sf, "materials": [{"name": "x", "kind": "ConeLight", "spread": 3}]})
+
+
+def test_moving_sphere_is_not_a_light_sampling_target():
+    sf = synth.cornell_json()
+    sf["objects"].append({"kind": "MovingSphere", "center_0": [0, 0, 0], "center_1": [0, 1, 0], "radius": 1,
+                          "priority": True})
+    with pytest.raises(clovers_amd.ClvError) as e:     # MovingSphere::random is unimplemented!() in the reference
+        clovers_amd.validate(load_scene(sf))
+    assert e.value.status == abi.EINVAL
 
 
 def test_validate_shapes():
