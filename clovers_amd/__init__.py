@@ -81,6 +81,15 @@ def load_library(path=LIB_PATH):
     lib.clv_wavelength_into_xyz.argtypes = [C.c_uint32, C.POINTER(C.c_float)]
     lib.clv_scene_validate.restype = C.c_int
     lib.clv_scene_validate.argtypes = [vp, vp]
+    lib.clv_scene_file_parse.restype = C.c_int
+    lib.clv_scene_file_parse.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp), C.c_char_p, C.c_uint32]
+    lib.clv_scene_file_desc.restype = vp
+    lib.clv_scene_file_desc.argtypes = [vp]
+    lib.clv_scene_file_free.argtypes = [vp]
+    lib.clv_write_png.restype = C.c_int
+    lib.clv_write_png.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_char_p, C.c_char_p]
+    lib.clv_write_exr.restype = C.c_int
+    lib.clv_write_exr.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_char_p]
     if lib.clv_abi_version() != abi.ABI_VERSION:
         raise ImportError("libclovers_hip.so ABI %d != %d" % (lib.clv_abi_version(), abi.ABI_VERSION))
     _lib = lib
@@ -111,6 +120,81 @@ def validate(scene):
     if st != abi.OK:
         raise ClvError(st, info.error.decode())
     return {k: getattr(info, k) for k, _ in abi.SceneInfo._fields_ if k != "error"}
+
+
+class SceneFile:
+    """A scene file parsed by the library (clv_scene_file_parse): usable
+    wherever a SceneDescription is (upload, validate); `objects`,
+    `materials` and `xforms` are numpy copies of its descriptor."""
+
+    def __init__(self, src, mesh_root=None, bvh_algorithm=abi.BVH_SAH):
+        lib = load_library()
+        if not src.lstrip().startswith("{"):
+            with open(src) as f:
+                src = f.read()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        st = lib.clv_scene_file_parse(src.encode(), mesh_root.encode() if mesh_root else None, bvh_algorithm,
+                                      C.byref(h), err, 512)
+        if st != abi.OK:
+            raise ClvError(st, err.value.decode())
+        self._lib, self.h = lib, h
+        d = abi.SceneDesc.from_address(lib.clv_scene_file_desc(h))
+        self._desc = d
+
+        def arr(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            return np.frombuffer((C.c_char * (n * dt.itemsize)).from_address(ptr), dt).copy()
+        self.objects = arr(d.objects, d.n_objects, abi.OBJECT_DTYPE)
+        self.materials = arr(d.materials, d.n_materials, abi.MATERIAL_DTYPE)
+        self.xforms = arr(d.xforms, d.n_xforms, abi.XFORM_DTYPE)
+        self.background = np.array(d.background[:], np.float32)
+        self.camera = {k: (list(getattr(d.camera, k)) if k in ("look_from", "look_at", "up") else getattr(d.camera, k))
+                       for k, _ in abi.CameraInit._fields_}
+
+    def as_ctypes(self):
+        return self._desc, self
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.clv_scene_file_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _image(img):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    if img.ndim != 3 or img.shape[2] != 3:
+        raise ValueError("expected an [H, W, 3] Xyz<E> image")
+    return img
+
+
+def write_png(img, path, comment=None):
+    """clv_write_png: Xyz<E> [H, W, 3] (row 0 = top) -> 8-bit sRGB PNG (write.rs:18-84)."""
+    img = _image(img)
+    st = load_library().clv_write_png(img.ctypes.data, img.shape[1], img.shape[0], os.fsencode(path),
+                                      comment.encode() if comment else None)
+    if st != abi.OK:
+        raise ClvError(st, "cannot write %s" % path)
+
+
+def write_exr(img, path):
+    """clv_write_exr: Xyz<E> [H, W, 3] -> OpenEXR, R/G/B = X/Y/Z (write.rs:86-125)."""
+    img = _image(img)
+    st = load_library().clv_write_exr(img.ctypes.data, img.shape[1], img.shape[0], os.fsencode(path))
+    if st != abi.OK:
+        raise ClvError(st, "cannot write %s" % path)
+
+
+def load_scene_file(src, mesh_root=None, bvh_algorithm=abi.BVH_SAH):
+    """Parse a scene file (path or JSON text) in the library; see SceneFile."""
+    return SceneFile(src, mesh_root, bvh_algorithm)
 
 
 class Renderer:

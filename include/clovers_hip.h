@@ -231,6 +231,34 @@ typedef struct {
 } clv_scene_info;
 clv_status clv_scene_validate(const clv_scene_desc *desc, clv_scene_info *info);
 
+/* Scene files, no device needed.  Parses the reference's scene-file JSON --
+ * SceneFile::initialize (clovers-cli/src/scenefile.rs:34-117 with
+ * json_scene.rs:12-27) and object_to_hitable (clovers/src/objects.rs:92-157),
+ * including STL / PLY meshes (objects/stl.rs:38-96, objects/ply.rs:37-103)
+ * -- into a descriptor owned by the returned handle, ready for
+ * clv_scene_upload / clv_scene_validate.  json: the file's text,
+ * NUL-terminated.  mesh_root: directory that relative mesh paths are read
+ * from (NULL: the working directory, as the reference).  bvh_algorithm: the
+ * clv_bvh_algorithm of the light-sampling tree.  Kinds the reference's
+ * schema has but this library does not render are CLV_EUNSUPPORTED; inputs
+ * on which the reference panics are CLV_EINVAL.  err (nullable) receives the
+ * reason. */
+typedef struct clv_scene_file clv_scene_file;
+clv_status clv_scene_file_parse(const char *json, const char *mesh_root, uint32_t bvh_algorithm,
+                                clv_scene_file **out, char *err, uint32_t err_len);
+/* The parsed descriptor; valid until clv_scene_file_free. */
+const clv_scene_desc *clv_scene_file_desc(const clv_scene_file *file);
+void clv_scene_file_free(clv_scene_file *file);
+
+/* Image files of a clv_render pixel buffer (W*H*3 floats, Xyz<E>, row 0 =
+ * top), no device needed.  clv_write_png == write::png (clovers-cli/src/
+ * write.rs:18-84): sRGB 8-bit (Bradford E -> D65, sRGB transfer function,
+ * clamped), with a tEXt "Comment" (comment, nullable) and "Software".
+ * clv_write_exr == write::exr (write.rs:86-125): float R/G/B = X/Y/Z with the
+ * XYZ chromaticities attribute, uncompressed scanlines. */
+clv_status clv_write_png(const float *xyz, uint32_t width, uint32_t height, const char *path, const char *comment);
+clv_status clv_write_exr(const float *xyz, uint32_t width, uint32_t height, const char *path);
+
 /* == draw_cpu::draw for RenderMode::PathTracing (draw_cpu.rs:26-146).
  * xyz_out: caller-allocated W*H*3 floats (row 0 = top, index = x + row*W),
  * the per-pixel mean over opts->samples.  xyz_sq_out (nullable): per-pixel
