@@ -86,6 +86,8 @@ def load_library(path=LIB_PATH):
     lib.clv_scene_file_desc.restype = vp
     lib.clv_scene_file_desc.argtypes = [vp]
     lib.clv_scene_file_free.argtypes = [vp]
+    lib.clv_render_debug.restype = C.c_int
+    lib.clv_render_debug.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp]
     lib.clv_write_png.restype = C.c_int
     lib.clv_write_png.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_char_p, C.c_char_p]
     lib.clv_write_exr.restype = C.c_int
@@ -328,6 +330,13 @@ class Renderer:
         ts = np.zeros((H, W), np.float32)
         self._check(self.lib.clv_primary_hits(self.h, scene_id, C.byref(opts), ids.ctypes.data, ts.ctypes.data))
         return ids, ts
+
+    def render_debug(self, scene_id, opts, mode):
+        """clv_render_debug: abi.MODE_NORMAL_MAP / MODE_BVH_TEST_COUNT /
+        MODE_PRIMITIVE_TEST_COUNT image, Xyz<E> [H, W, 3]."""
+        img = np.zeros((opts.height, opts.width, 3), np.float32)
+        self._check(self.lib.clv_render_debug(self.h, scene_id, C.byref(opts), int(mode), img.ctypes.data))
+        return img
 
     def counters(self):
         c = abi.Counters()

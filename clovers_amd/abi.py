@@ -28,6 +28,8 @@ TEX_SOLID, TEX_ILLUMINANT_D65, TEX_ILLUMINANT_D50, TEX_SPATIAL_CHECKER, TEX_SURF
 # BVH algorithms (bvh.rs:32-40)
 BVH_LAM, BVH_SAH = 0, 1
 GUIDE_OFF, GUIDE_ON = 0, 1
+# render modes (render.rs:52-62)
+MODE_PATH_TRACING, MODE_NORMAL_MAP, MODE_BVH_TEST_COUNT, MODE_PRIMITIVE_TEST_COUNT = range(4)
 
 OBJECT_DTYPE = np.dtype([
     ("kind", "<u4"), ("material", "<u4"), ("priority", "<u4"), ("xform", "<u4"),

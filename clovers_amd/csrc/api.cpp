@@ -305,6 +305,33 @@ clv_status arena_append(clv_ctx *c, DevArray &a, const void *src, size_t bytes) 
   return CLV_OK;
 }
 
+/* r = P^-1 v for a 3x3 P (Cramer's rule) */
+void inv3_apply(const double P[3][3], const double v[3], double r[3]) {
+  const double det = P[0][0] * (P[1][1] * P[2][2] - P[1][2] * P[2][1]) - P[0][1] * (P[1][0] * P[2][2] - P[1][2] * P[2][0]) +
+                     P[0][2] * (P[1][0] * P[2][1] - P[1][1] * P[2][0]);
+  for (int k = 0; k < 3; k++) {
+    double Q[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) Q[i][j] = j == k ? v[i] : P[i][j];
+    r[k] = (Q[0][0] * (Q[1][1] * Q[2][2] - Q[1][2] * Q[2][1]) - Q[0][1] * (Q[1][0] * Q[2][2] - Q[1][2] * Q[2][0]) +
+            Q[0][2] * (Q[1][0] * Q[2][1] - Q[1][1] * Q[2][0])) / det;
+  }
+}
+
+/* Bradford adaptation XYZ(white w) -> XYZ(E): B^-1 diag((B 1) / (B w)) B */
+void bradford(const double B[3][3], const double w[3], double A[3][3]) {
+  double d[3], Bi[3][3];
+  for (int i = 0; i < 3; i++) d[i] = (B[i][0] + B[i][1] + B[i][2]) / (B[i][0] * w[0] + B[i][1] * w[1] + B[i][2] * w[2]);
+  for (int j = 0; j < 3; j++) {
+    double e[3] = {0, 0, 0}, col[3];
+    e[j] = 1.0;
+    inv3_apply(B, e, col);
+    for (int i = 0; i < 3; i++) Bi[i][j] = col[i];
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) A[i][j] = Bi[i][0] * d[0] * B[0][j] + Bi[i][1] * d[1] * B[1][j] + Bi[i][2] * d[2] * B[2][j];
+}
+
 DevScene arena_view(const clv_ctx *c) {
   DevScene d;
   memset(&d, 0, sizeof(d));
@@ -1279,6 +1306,80 @@ clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   HIPCHK(c, hipMemcpyAsync(ids, c->ids, sizeof(uint32_t) * WH, hipMemcpyDeviceToHost, c->stream));
   if (ts) HIPCHK(c, hipMemcpyAsync(ts, c->ts, sizeof(float) * WH, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLV_OK;
+}
+
+clv_status clv_render_debug(clv_ctx *c, uint32_t scene_id, const clv_render_opts *o, uint32_t mode, float *xyz_out) {
+  if (!c) return CLV_EINVAL;
+  CHK(check_opts(c, o));
+  if (!xyz_out) return fail(c, CLV_EINVAL, "xyz_out is NULL");
+  if (mode < CLV_MODE_NORMAL_MAP || mode > CLV_MODE_PRIMITIVE_TEST_COUNT)
+    return fail(c, CLV_EINVAL, "mode is not a debug mode (path tracing is clv_render)");
+  auto it = c->scenes.find(scene_id);
+  if (it == c->scenes.end()) return fail(c, CLV_EINVAL, "unknown scene id");
+  HIPCHK(c, hipSetDevice(c->device));
+  const uint32_t W = o->width, H = o->height, WH = W * H;
+  CHK(ensure_paths(c, WH));
+  CHK(ensure_pixels(c, WH));
+  CHK(ensure_tables(c, 1, 1));
+  const uint32_t blocks = c->trace_blocks;
+  CHK(ensure_spill(c, trace_spill_words(blocks)));
+  DevJob jb = make_job(it->second, it->second->camera, o);
+  HIPCHK(c, hipMemcpyAsync(c->jobs, &jb, sizeof(jb), hipMemcpyHostToDevice, c->stream));
+  DevScene ds = arena_view(c);
+  ds.has_xforms = it->second->has_xforms ? 1u : 0u;
+  ds.has_ext = it->second->has_ext ? 1u : 0u;
+  DebugArgs da;
+  memset(&da, 0, sizeof(da));
+  da.width = W;
+  da.height = H;
+  da.sample = o->sample_offset;
+  da.mode = mode;
+  /* LinSrgb -> Xyz<E>: the sRGB primaries' matrix, then Bradford D65 -> E */
+  {
+    const double xy[3][2] = {{0.64, 0.33}, {0.30, 0.60}, {0.15, 0.06}};
+    const double wt[3] = {0.95047, 1.0, 1.08883};
+    const double B[3][3] = {{0.8951, 0.2664, -0.1614}, {-0.7502, 1.7135, 0.0367}, {0.0389, -0.0685, 1.0296}};
+    double P[3][3], M[3][3], A[3][3];
+    for (int k = 0; k < 3; k++) {
+      P[0][k] = xy[k][0] / xy[k][1];
+      P[1][k] = 1.0;
+      P[2][k] = (1.0 - xy[k][0] - xy[k][1]) / xy[k][1];
+    }
+    double S[3];
+    inv3_apply(P, wt, S);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) M[i][j] = P[i][j] * S[j];
+    bradford(B, wt, A);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) da.lin_to_xyz[3 * i + j] = (float)(A[i][0] * M[0][j] + A[i][1] * M[1][j] + A[i][2] * M[2][j]);
+  }
+  launch_generate_debug(c->stream, ds, c->pb, da);
+  launch_zero_u64(c->stream, c->stats, 8);
+  TraceArgs ta;
+  memset(&ta, 0, sizeof(ta));
+  ta.counts = c->counts;
+  ta.n_direct = WH;
+  ta.ids = c->ids;
+  ta.ts = c->ts;
+  ta.spill = c->spill;
+  ta.stats = (unsigned long long *)c->stats;
+  uint32_t *rc = reinterpret_cast<uint32_t *>(c->acc64);  /* 2 words per ray within the 6 the pixel holds */
+  ta.ray_counts = rc;
+  launch_primary_counting(c->stream, ds, c->pb, ta, blocks);
+  launch_debug_color(c->stream, ds, c->pb, da, rc, c->out);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(xyz_out, c->out, sizeof(float) * 3 * (size_t)WH, hipMemcpyDeviceToHost, c->stream));
+  uint64_t st[8];
+  HIPCHK(c, hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  clv_counters cn;
+  memset(&cn, 0, sizeof(cn));
+  cn.samples = WH;
+  cn.rays = WH;
+  cn.node_visits = st[0];
+  cn.prim_tests = st[1];
+  c->counters = cn;
   return CLV_OK;
 }
 

@@ -86,6 +86,7 @@ struct TraceArgs {
   unsigned long long *stats;   /* counting mode: node visits, primitive tests */
   uint32_t *ids;               /* primary mode outputs */
   float *ts;
+  uint32_t *ray_counts;        /* primary counting mode (nullable): per ray node visits, primitive tests */
   uint32_t lds_node_base;      /* LDS variant: nodes [base, base + lds_nodes) are cached in LDS */
   uint32_t lds_nodes;
   uint32_t lds_unit_base;      /* LDS variant: primitive float4 units [base, base + lds_units) are cached */
@@ -139,6 +140,15 @@ void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const 
                    bool counting, int variant);
 void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                     int variant);
+/* debug render modes: one ray per pixel, closest hit with per-ray counts, colours */
+struct DebugArgs {
+  uint32_t width, height, sample, mode;   /* mode: clv_render_mode */
+  float lin_to_xyz[9];                    /* linear sRGB -> Xyz<E>, row-major */
+};
+void launch_generate_debug(hipStream_t st, const DevScene &sc, const PathBuf &pb, const DebugArgs &da);
+void launch_primary_counting(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks);
+void launch_debug_color(hipStream_t st, const DevScene &sc, const PathBuf &pb, const DebugArgs &da,
+                        const uint32_t *ray_counts, float *out);
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
                   uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga);
 /* learned-proposal lobes for the Lambertian queue of the current bounce */

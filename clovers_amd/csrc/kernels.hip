@@ -609,6 +609,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathB
     L.path = 0;
     if (active) lane_start(L, pb, MODE == 0 ? ta.qin[i] : i);
     else L.word = CHILD_EMPTY;
+    const uint32_t v0 = nvisit, t0 = ntest;
     if (WW) {
       /* while-while: the wave descends interior nodes until every lane holds
        * a leaf (or is done), then all lanes test their leaf together */
@@ -652,6 +653,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathB
       } else {
         ta.ids[L.path] = sc.info[as_u(sc.prims[L.best + 1u].w)].report_id;
         ta.ts[L.path] = L.tb;
+      }
+      pb.hit_t[L.path] = L.tb;
+      pb.hit_p[L.path] = L.best;
+      if (COUNT && ta.ray_counts) {
+        ta.ray_counts[2u * L.path] = nvisit - v0;
+        ta.ray_counts[2u * L.path + 1u] = ntest - t0;
       }
     }
   }
@@ -1307,6 +1314,63 @@ __global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, Gu
   }
 }
 
+/* ------------------------------------------------------- debug modes --- */
+/* One camera ray per pixel through the pixel's corner (x/W, y/H) with no lens
+ * offset, as render_pixel_normalmap / _bvhtestcount / _primitivetestcount
+ * (draw_cpu.rs:148-207); the pixel's stream draws the wavelength, then the time. */
+__global__ __launch_bounds__(256) void k_generate_debug(DevScene sc, PathBuf pb, DebugArgs da) {
+  const uint32_t W = da.width, H = da.height, n = W * H;
+  const DevJob &jb = sc.jobs[0];
+  const DevCamera &c = jb.cam;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t r = i / W, x = i - r * W, y = H - 1u - r;
+    uint64_t s = pcg_for_path(jb.seed, x + y * W, da.sample);
+    const uint32_t wl = (uint32_t)(((uint64_t)pcg_next(s) * 400ull) >> 32);
+    const float time = rnd(s);
+    const float u = (float)x / (float)W, v = (float)y / (float)H;
+    V org = mk(c.origin[0], c.origin[1], c.origin[2]);
+    V dir = mk(c.llc[0], c.llc[1], c.llc[2]) + u * mk(c.horizontal[0], c.horizontal[1], c.horizontal[2]) +
+            v * mk(c.vertical[0], c.vertical[1], c.vertical[2]) - org;
+    dir = unit(dir);
+    pb.ray_o[i] = make_float4(org.x, org.y, org.z, __uint_as_float(jb.root));
+    pb.ray_d[i] = make_float4(dir.x, dir.y, dir.z, time);
+    pb.meta[i] = wl;
+  }
+}
+
+/* normal_to_color (normals.rs:21-33) of the first hit, or the count ramp of
+ * bvh_testcount_to_color / primitive_testcount_to_color
+ * (debug_visualizations.rs:17-33, 45-61), as linear sRGB -> Xyz<E> */
+__global__ __launch_bounds__(256) void k_debug_color(DevScene sc, PathBuf pb, DebugArgs da, const uint32_t *rc,
+                                                     float *out) {
+  const uint32_t n = da.width * da.height;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float r, g, b;
+    if (da.mode == 1u) {
+      const uint32_t off = pb.hit_p[i];
+      if (off == MISS_WORD) {
+        r = g = b = 0.0f;
+      } else {
+        const float4 o4 = pb.ray_o[i], d4 = pb.ray_d[i];
+        const Hit h = hit_record<false>(sc, off, xyz(o4), xyz(d4), d4.w, pb.hit_t[i]);
+        const float nx = -h.n.x, ny = h.n.y, nz = -h.n.z;
+        r = 0.5f + 0.5f * nx;
+        g = 0.5f + 0.5f * ny;
+        b = nz < 0.0f ? 0.0f : nz;
+      }
+    } else {
+      const uint32_t k = rc[2u * i + (da.mode == 3u ? 1u : 0u)];
+      if (k <= 255u) r = g = b = (float)k / 255.0f;
+      else if (k <= 511u) r = 1.0f, g = 1.0f, b = 0.0f;
+      else if (k <= 1023u) r = 1.0f, g = 0.5f, b = 0.0f;
+      else r = 1.0f, g = 0.0f, b = 0.0f;
+    }
+    const float *m = da.lin_to_xyz;
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[3u * i + k] = m[3 * k] * r + m[3 * k + 1] * g + m[3 * k + 2] * b;
+  }
+}
+
 /* per-sample estimator (draw_cpu.rs:130-142) and per-pixel sums in sample
  * order; one thread per (pass item, pixel) */
 __global__ __launch_bounds__(256) void k_resolve(PathBuf pb, ResolveParams rp, const float4 *cie) {
@@ -1467,6 +1531,18 @@ void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const
     hipLaunchKernelGGL((k_trace<false, MODE_PRIMARY>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
   else
     hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+}
+void launch_generate_debug(hipStream_t st, const DevScene &sc, const PathBuf &pb, const DebugArgs &da) {
+  hipLaunchKernelGGL(k_generate_debug, dim3(grid_for(da.width * da.height, 256, 8192)), dim3(256), 0, st, sc, pb, da);
+}
+void launch_primary_counting(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta,
+                             uint32_t blocks) {
+  hipLaunchKernelGGL((k_trace_simple<true, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+}
+void launch_debug_color(hipStream_t st, const DevScene &sc, const PathBuf &pb, const DebugArgs &da,
+                        const uint32_t *ray_counts, float *out) {
+  hipLaunchKernelGGL(k_debug_color, dim3(grid_for(da.width * da.height, 256, 8192)), dim3(256), 0, st, sc, pb, da,
+                     ray_counts, out);
 }
 void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
                   uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga) {

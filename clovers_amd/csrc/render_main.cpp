@@ -6,8 +6,9 @@
  *   clovers-render -i scenes/cornell.json -w 1024 -h 1024 -s 64 -d 64 \
  *                  -f png exr -o renders/cornell [--bvh sah|lam] [--seed N] [--guiding]
  *
- * Options and defaults are RenderOptions' (render.rs:17-49); path tracing is
- * the only mode and `random` the only sampler (the others are reported as
+ * Options and defaults are RenderOptions' (render.rs:17-49), with the modes
+ * path-tracing, normal-map, bvh-test-count and primitive-test-count
+ * (render.rs:52-62); `random` is the only sampler (blue is reported as
  * unsupported).  The output path has no extension, as in the reference; the
  * default is renders/<unix time>.  Mesh paths in the scene file are relative
  * to the working directory, or to --mesh-root.
@@ -29,7 +30,8 @@ int usage(const char *msg) {
   if (msg) fprintf(stderr, "error: %s\n", msg);
   fprintf(stderr,
           "usage: clovers-render -i <scene.json> [-o <output without extension>] [-w 1024] [-h 1024]\n"
-          "                      [-s 64] [-d 64] [-m path-tracing] [--sampler random] [--bvh sah|lam]\n"
+          "                      [-s 64] [-d 64] [-m path-tracing|normal-map|bvh-test-count|primitive-test-count]\n"
+          "                      [--sampler random] [--bvh sah|lam]\n"
           "                      [-f png [exr]] [--seed N] [--guiding] [--mesh-root DIR] [--device N]\n");
   return 2;
 }
@@ -85,7 +87,12 @@ int main(int argc, char **argv) {
     else return usage(("unknown option " + a).c_str());
   }
   if (input.empty()) return usage("--input is required");
-  if (mode != "path-tracing") return usage("only the path-tracing mode is supported");
+  uint32_t mode_id;
+  if (mode == "path-tracing") mode_id = CLV_MODE_PATH_TRACING;
+  else if (mode == "normal-map") mode_id = CLV_MODE_NORMAL_MAP;
+  else if (mode == "bvh-test-count") mode_id = CLV_MODE_BVH_TEST_COUNT;
+  else if (mode == "primitive-test-count") mode_id = CLV_MODE_PRIMITIVE_TEST_COUNT;
+  else return usage("unknown mode");
   if (sampler != "random") return usage("only the random sampler is supported");
   if (bvh != "sah" && bvh != "lam") return usage("--bvh is sah or lam");
   if (formats.empty()) formats.push_back("png");
@@ -132,7 +139,8 @@ int main(int argc, char **argv) {
   printf("Rendering %s at %ux%u, %u samples per pixel, max depth %u\n", input.c_str(), width, height, samples,
          max_depth);
   auto t0 = std::chrono::steady_clock::now();
-  st = clv_render(ctx, sid, &o, img.data(), nullptr);
+  st = mode_id == CLV_MODE_PATH_TRACING ? clv_render(ctx, sid, &o, img.data(), nullptr)
+                                       : clv_render_debug(ctx, sid, &o, mode_id, img.data());
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st != CLV_OK) {
     fprintf(stderr, "error: render: %s\n", clv_last_error(ctx));
@@ -144,9 +152,14 @@ int main(int argc, char **argv) {
   printf("Rendering finished in %.3f s (%.1f Msamples/s)\n", secs, msps);
   char comment[1024];
   snprintf(comment, sizeof(comment),
-           "Rendered with clovers_amd. Scene file %s rendered using the PathTracing rendering mode at %ux%u "
-           "resolution, %u samples per pixel, %u max ray bounce depth. Rendering finished in %.3f s on one GPU.",
-           input.c_str(), width, height, samples, max_depth, secs);
+           "Rendered with clovers_amd. Scene file %s rendered using the %s rendering mode at %ux%u resolution%s "
+           "Rendering finished in %.3f s on one GPU.",
+           input.c_str(), mode.c_str(), width, height,
+           mode_id == CLV_MODE_PATH_TRACING
+               ? (", " + std::to_string(samples) + " samples per pixel, " + std::to_string(max_depth) +
+                  " max ray bounce depth.").c_str()
+               : ".",
+           secs);
   {
     std::error_code ec;
     std::filesystem::path parent = std::filesystem::path(output).parent_path();

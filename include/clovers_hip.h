@@ -174,6 +174,15 @@ typedef struct {
 #define CLV_FLAG_GUIDE_FROZEN 4u  /* guided jobs sample with the network but never train it (ablation) */
 #define CLV_FLAG_GUIDE_KEEP 8u    /* guided jobs start from the context's current network instead of a fresh one */
 
+/* RenderMode (clovers-cli/src/render.rs:52-62). */
+typedef enum {
+  CLV_MODE_PATH_TRACING = 0,          /* clv_render / clv_render_batch */
+  CLV_MODE_NORMAL_MAP = 1,            /* first-hit normal as a colour (draw_cpu.rs:148-166, normals.rs:9-33) */
+  CLV_MODE_BVH_TEST_COUNT = 2,        /* BVH nodes visited by the pixel's ray, as a colour ramp */
+  CLV_MODE_PRIMITIVE_TEST_COUNT = 3   /* primitives tested by the pixel's ray, as a colour ramp
+                                         (draw_cpu.rs:168-207, debug_visualizations.rs:9-61) */
+} clv_render_mode;
+
 /* Counters of the last clv_render / clv_primary_hits on a context. */
 typedef struct {
   uint64_t samples;          /* camera paths traced */
@@ -266,6 +275,16 @@ clv_status clv_write_exr(const float *xyz, uint32_t width, uint32_t height, cons
  * Blocking. */
 clv_status clv_render(clv_ctx *ctx, uint32_t scene_id, const clv_render_opts *opts,
                       float *xyz_out, float *xyz_sq_out);
+
+/* The debug render modes (CLV_MODE_NORMAL_MAP, _BVH_TEST_COUNT,
+ * _PRIMITIVE_TEST_COUNT): one ray per pixel through the pixel's corner with
+ * no lens offset, colours as the reference's (linear sRGB -> Xyz<E>), same
+ * output layout as clv_render.  Counts are this library's: 4-wide BVH nodes
+ * fetched and primitives tested (the reference counts its binary tree), with
+ * the reference's ramp: grey n/255 up to 255, yellow to 511, orange to 1023,
+ * red beyond.  opts: width, height, seed, sample_offset (the stream index). */
+clv_status clv_render_debug(clv_ctx *ctx, uint32_t scene_id, const clv_render_opts *opts, uint32_t mode,
+                            float *xyz_out);
 
 /* One job of a batched render: one view of one uploaded scene.  No reference
  * counterpart -- the reference renders one scene per process run

@@ -1290,6 +1290,36 @@ int orc_primary_hits(const orc_scene *o, const clv_render_opts *opts, uint32_t *
   return CLV_OK;
 }
 
+/* render_pixel_normalmap (draw_cpu.rs:148-166) + normal_map / normal_to_color
+ * (normals.rs:9-33): one ray per pixel through (x/W, y/H), no lens offset, the
+ * stream drawing the wavelength and then the time; linear sRGB per pixel, row
+ * 0 = top (the conversion to Xyz<E> is the caller's). */
+int orc_normal_map(const orc_scene *o, const clv_render_opts *opts, float *rgb) {
+  if (!o || !opts || !rgb || !opts->width || !opts->height) return CLV_EINVAL;
+  const Scene &s = *o->s;
+  Camera cam = make_camera(s.cam_init, (Float)opts->width / (Float)opts->height);
+  for (uint32_t r = 0; r < opts->height; r++) {
+    uint32_t y = opts->height - 1 - r;
+    for (uint32_t x = 0; x < opts->width; x++) {
+      orc_pcg rng = orc_pcg_for_path(opts->seed, x + y * opts->width, opts->sample_offset);
+      int wl = orc_wavelength(&rng);
+      Float time = orc_uniform(&rng);
+      Ray ray = get_ray(cam, (Float)x / (Float)opts->width, (Float)y / (Float)opts->height, 0.0f, 0.0f, time, wl);
+      HitRecord rec;
+      float *px = rgb + 3 * ((size_t)r * opts->width + x);
+      if (!s.bvh_root->hit(ray, EPSILON_SHADOW_ACNE, FLT_MAX, &rec)) {
+        px[0] = px[1] = px[2] = 0.0f;
+        continue;
+      }
+      V3 n = v3(-rec.normal.x, rec.normal.y, -rec.normal.z);
+      px[0] = 0.5f + 0.5f * n.x;
+      px[1] = 0.5f + 0.5f * n.y;
+      px[2] = n.z < 0.0f ? 0.0f : n.z;
+    }
+  }
+  return CLV_OK;
+}
+
 /* One path: the 4 spectral radiances trace() returns and the hero wavelength. */
 int orc_trace_sample(const orc_scene *o, const clv_render_opts *opts, uint32_t x, uint32_t y, uint32_t sample,
                      float out4[4], int *hero) {

@@ -44,6 +44,8 @@ def _bind(lib):
                                C.c_void_p]
     lib.orc_primary_hits.restype = C.c_int
     lib.orc_primary_hits.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.orc_normal_map.restype = C.c_int
+    lib.orc_normal_map.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     lib.orc_trace_sample.restype = C.c_int
     lib.orc_trace_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, fp,
                                      C.POINTER(C.c_int)]
@@ -131,6 +133,14 @@ class OracleScene:
         if st != 0:
             raise RuntimeError("orc_primary_hits: %d" % st)
         return ids, ts
+
+    def normal_map(self, opts):
+        """linear sRGB [H, W, 3] of the normal-map mode"""
+        rgb = np.zeros((opts.height, opts.width, 3), np.float32)
+        st = self.lib.orc_normal_map(self.h, C.byref(opts), rgb.ctypes.data)
+        if st != 0:
+            raise RuntimeError("orc_normal_map: %d" % st)
+        return rgb
 
     def trace_sample(self, opts, x, y, sample):
         out = (C.c_float * 4)()

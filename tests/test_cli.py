@@ -22,7 +22,7 @@ def run(*args, **kw):
 
 @pytest.mark.parametrize("args,code", [
     ((), 2),
-    (("-i", "x.json", "-m", "normal-map"), 2),
+    (("-i", "x.json", "-m", "wireframe"), 2),
     (("-i", "x.json", "--sampler", "blue"), 2),
     (("-i", "x.json", "-f", "jpg"), 2),
     (("-i", "x.json", "-w", "abc"), 2),
@@ -60,3 +60,10 @@ def test_render_scene_file(tmp_path, renderer):
     px, texts = read_png(str(out) + ".png")
     assert px.shape == (64, 96, 3) and px.max() > 0
     assert b"8 samples per pixel" in texts[b"Comment"]
+    r = run("-i", p, "-w", 64, "-h", 48, "-m", "normal-map", "-f", "exr", "-o", tmp_path / "n", "--seed", 3)
+    assert r.returncode == 0, r.stderr
+    nm, _ = read_exr(str(tmp_path / "n") + ".exr")
+    sid = renderer.upload(clovers_amd.load_scene_file(str(p)))
+    want = renderer.render_debug(sid, abi.render_opts(64, 48, 1, 1, seed=3), abi.MODE_NORMAL_MAP)
+    renderer.free(sid)
+    assert np.array_equal(nm, want)

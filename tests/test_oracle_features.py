@@ -75,3 +75,28 @@ def test_moving_sphere_at_rest_equals_sphere():
     ib, _, _ = po.OracleScene(b).render(o)
     assert np.array_equal(ia, ib)
     assert ia.max() > 0
+
+
+def test_normal_map_closed_form():
+    """normal_to_color (normals.rs:21-33) of the unit sphere seen from +z:
+    the oriented normal of the hit point of each pixel's corner ray
+    (draw_cpu.rs:148-166), and (0.5, 0.5, 0) for the camera-facing wall."""
+    W, H = 40, 30
+    sphere = S.checker_sphere(8.0)
+    rgb = po.OracleScene(sphere).normal_map(abi.render_opts(W, H, 1, 1, seed=4))
+    half = np.tan(np.radians(15.0))
+    rows, cols = np.mgrid[0:H, 0:W]
+    u, v = cols / W, (H - 1 - rows) / H
+    d = np.stack([(2 * u - 1) * half * (W / H), (2 * v - 1) * half, -np.ones_like(u, dtype=np.float64)], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    o = np.array(S.CAM["look_from"], np.float64)
+    b = d @ o
+    disc = b * b - (o @ o - 1.0)
+    hit = disc > 1e-3
+    t = -b - np.sqrt(np.maximum(disc, 0))
+    n = o + t[..., None] * d
+    want = np.stack([0.5 - 0.5 * n[..., 0], 0.5 + 0.5 * n[..., 1], np.maximum(-n[..., 2], 0)], -1)
+    assert np.allclose(rgb[hit], want[hit], atol=1e-5)
+    assert np.all(rgb[disc < -1e-3] == 0)
+    wall = po.OracleScene(S.emitter_wall({"kind": "DiffuseLight"})).normal_map(abi.render_opts(W, H, 1, 1))
+    assert np.all(wall == np.array([0.5, 0.5, 0.0], np.float32))
