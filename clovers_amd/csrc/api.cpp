@@ -101,8 +101,7 @@ struct clv_ctx {
   DevArray arena[AR_COUNT];
   /* path state */
   uint32_t cap_paths = 0;
-  PathBuf pb;
-  uint32_t *queue[2] = {nullptr, nullptr};
+  PathBuf pb;                /* the view of even bounces; odd bounces swap the two entry sets */
   uint32_t *shadeq[MC_COUNT] = {nullptr};
   /* accumulators */
   uint32_t cap_pixels = 0;
@@ -173,14 +172,28 @@ void dfree(void *p) {
 }
 
 void free_paths(clv_ctx *c) {
-  dfree(c->pb.ray_o); dfree(c->pb.ray_d); dfree(c->pb.thr); dfree(c->pb.rad);
-  dfree(c->pb.rng); dfree(c->pb.meta); dfree(c->pb.hit_t); dfree(c->pb.hit_p);
-  for (int i = 0; i < 2; i++) dfree(c->queue[i]);
+  PathBuf &b = c->pb;
+  dfree(b.ray_o); dfree(b.ray_d); dfree(b.thr); dfree(b.rng); dfree(b.meta); dfree(b.slot);
+  dfree(b.n_ray_o); dfree(b.n_ray_d); dfree(b.n_thr); dfree(b.n_rng); dfree(b.n_meta); dfree(b.n_slot);
+  dfree(b.hit_t); dfree(b.hit_p); dfree(b.rad); dfree(b.hero);
   for (uint32_t k = 0; k < MC_COUNT; k++) dfree(c->shadeq[k]);
   memset(&c->pb, 0, sizeof(c->pb));
-  c->queue[0] = c->queue[1] = nullptr;
   for (uint32_t k = 0; k < MC_COUNT; k++) c->shadeq[k] = nullptr;
   c->cap_paths = 0;
+}
+
+/* the path state of bounce `depth`: entries in one set, survivors to the other */
+PathBuf bounce_view(const PathBuf &pb, uint32_t depth) {
+  PathBuf v = pb;
+  if (depth & 1u) {
+    std::swap(v.ray_o, v.n_ray_o);
+    std::swap(v.ray_d, v.n_ray_d);
+    std::swap(v.thr, v.n_thr);
+    std::swap(v.rng, v.n_rng);
+    std::swap(v.meta, v.n_meta);
+    std::swap(v.slot, v.n_slot);
+  }
+  return v;
 }
 
 clv_status ensure_spill(clv_ctx *c, size_t words) {
@@ -228,15 +241,23 @@ void fill_trace_args(TraceArgs *ta, const TraceGeom &g) {
 clv_status ensure_paths(clv_ctx *c, uint32_t n) {
   if (n <= c->cap_paths) return CLV_OK;
   free_paths(c);
-  HIPCHK(c, dalloc(&c->pb.ray_o, n));
-  HIPCHK(c, dalloc(&c->pb.ray_d, n));
-  HIPCHK(c, dalloc(&c->pb.thr, n));
-  HIPCHK(c, dalloc(&c->pb.rad, n));
-  HIPCHK(c, dalloc(&c->pb.rng, n));
-  HIPCHK(c, dalloc(&c->pb.meta, n));
-  HIPCHK(c, dalloc(&c->pb.hit_t, n));
-  HIPCHK(c, dalloc(&c->pb.hit_p, n));
-  for (int i = 0; i < 2; i++) HIPCHK(c, dalloc(&c->queue[i], n));
+  PathBuf &b = c->pb;
+  HIPCHK(c, dalloc(&b.ray_o, n));
+  HIPCHK(c, dalloc(&b.ray_d, n));
+  HIPCHK(c, dalloc(&b.thr, n));
+  HIPCHK(c, dalloc(&b.rng, n));
+  HIPCHK(c, dalloc(&b.meta, n));
+  HIPCHK(c, dalloc(&b.slot, n));
+  HIPCHK(c, dalloc(&b.n_ray_o, n));
+  HIPCHK(c, dalloc(&b.n_ray_d, n));
+  HIPCHK(c, dalloc(&b.n_thr, n));
+  HIPCHK(c, dalloc(&b.n_rng, n));
+  HIPCHK(c, dalloc(&b.n_meta, n));
+  HIPCHK(c, dalloc(&b.n_slot, n));
+  HIPCHK(c, dalloc(&b.hit_t, n));
+  HIPCHK(c, dalloc(&b.hit_p, n));
+  HIPCHK(c, dalloc(&b.rad, n));
+  HIPCHK(c, dalloc(&b.hero, n));
   for (uint32_t k = 0; k < MC_COUNT; k++) HIPCHK(c, dalloc(&c->shadeq[k], n));
   c->cap_paths = n;
   return CLV_OK;
@@ -594,7 +615,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     gp.n_items = pl.first[p + 1] - pl.first[p];
     gp.paths = pl.paths[p];
     launch_set_count(c->stream, c->counts, CNT_QA, gp.paths);
-    launch_generate(c->stream, ds, pb, gp, c->queue[0]);
+    launch_generate(c->stream, ds, pb, gp);
     if (any_guide) {
       /* plain sampling until the network has had a training round (or, frozen, always the network) */
       ga.alpha = (gs.trained || frozen) ? gs.alpha : 0.0f;
@@ -602,13 +623,13 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
       ga.stamp = ++gs.stamp;
     }
     for (uint32_t depth = 0; depth <= pl.depth[p]; depth++) {
+      const PathBuf bv = bounce_view(pb, depth);
       uint32_t in = depth & 1u, out = in ^ 1u;
       uint32_t in_slot = in ? CNT_QB : CNT_QA, out_slot = out ? CNT_QB : CNT_QA;
       launch_reset_counts(c->stream, c->counts, in_slot, out_slot, c->stats);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev], c->stream));
       TraceArgs ta;
       memset(&ta, 0, sizeof(ta));
-      ta.qin = c->queue[in];
       ta.counts = c->counts;
       ta.in_slot = in_slot;
       ta.sq = sq;
@@ -616,17 +637,17 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
       ta.stats = (unsigned long long *)c->stats;
       fill_trace_args(&ta, geom);
       if (c->trace_variant == TRACE_LDS)
-        launch_trace_lds(c->stream, ds, pb, ta, geom.blocks, geom.lds_bytes, counting, 0);
+        launch_trace_lds(c->stream, ds, bv, ta, geom.blocks, geom.lds_bytes, counting, 0);
       else
-        launch_extend(c->stream, ds, pb, ta, geom.blocks, counting, c->trace_variant);
+        launch_extend(c->stream, ds, bv, ta, geom.blocks, counting, c->trace_variant);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
       ev += timing ? 2 : 0;
       if (any_guide && ga.alpha > 0.0f) {
         hipEvent_t end = timing ? guide_event(gev_infer) : nullptr;
-        launch_guide_infer(c->stream, ds, pb, ga, sq.q[MC_LAMBERTIAN], sq.counts + MC_LAMBERTIAN, gp.paths);
+        launch_guide_infer(c->stream, ds, bv, ga, sq.q[MC_LAMBERTIAN], sq.counts + MC_LAMBERTIAN, gp.paths);
         if (end) (void)hipEventRecord(end, c->stream);
       }
-      launch_shade(c->stream, ds, pb, sq, c->queue[out], c->counts, out_slot, gp.paths, c->grid_cap, ga);
+      launch_shade(c->stream, ds, bv, sq, c->counts, out_slot, gp.paths, c->grid_cap, ga);
     }
     rp.items = gp.items;
     rp.n_items = gp.n_items;
@@ -665,7 +686,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     add(gf, sizeof(gf));
     add(&geom, sizeof(geom));
     add(&pb, sizeof(pb));
-    const void *ptrs[] = {c->queue[0], c->queue[1], c->shadeq[0], c->shadeq[1], c->shadeq[2], c->shadeq[3],
+    const void *ptrs[] = {c->shadeq[0], c->shadeq[1], c->shadeq[2], c->shadeq[3],
                           c->shadeq[4], c->acc32, c->acc64, c->accsq, c->out, c->out_sq, c->counts, c->stats,
                           c->spill, c->cie, c->jobs, c->items, gs.lobes, gs.rec, gs.gcnt, gs.list, gs.bcount,
                           gs.bweight, gs.totals, gs.P, gs.M, gs.V, gs.Wh, gs.partial};
@@ -1286,7 +1307,7 @@ clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   gp.items = c->items;
   gp.n_items = 1;
   gp.paths = WH;
-  launch_generate(c->stream, ds, c->pb, gp, c->queue[0]);
+  launch_generate(c->stream, ds, c->pb, gp);
   launch_set_count(c->stream, c->counts, CNT_FETCH, 0u);
   TraceArgs ta;
   memset(&ta, 0, sizeof(ta));

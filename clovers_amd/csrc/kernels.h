@@ -44,16 +44,30 @@ struct DevScene {
   uint32_t has_ext;       /* some scene of the batch has checker textures or cone lights */
 };
 
-/* structure-of-arrays path state, one slot per in-flight sample */
+/* Structure-of-arrays path state.  A pass has one slot per sample (a pixel
+ * and sample index of one job).  The live paths of a bounce are kept
+ * compacted: entry e of the moving arrays is the e-th live path, so the
+ * traversal reads its rays contiguously; shading writes the survivors,
+ * compacted again, into the other half of a ping-pong pair (n_*).  What a
+ * path accumulates (radiance) and what the estimator needs (hero index) stay
+ * at the path's slot. */
 struct PathBuf {
-  float4 *ray_o;    /* origin */
-  float4 *ray_d;    /* unit direction */
+  /* by entry, this bounce */
+  float4 *ray_o;    /* origin, .w = the root node of the path's scene */
+  float4 *ray_d;    /* unit direction, .w = the ray's time */
   float4 *thr;      /* throughput, 4 hero wavelengths */
-  float4 *rad;      /* accumulated radiance, 4 hero wavelengths */
   uint64_t *rng;    /* PCG32 state */
   uint32_t *meta;   /* META_*: hero wavelength index, depth, job */
+  uint32_t *slot;   /* the path's slot */
   float *hit_t;
   uint32_t *hit_p;  /* primitive record offset of the closest hit, ~0u on a miss */
+  /* by entry, next bounce (shading writes the survivors here) */
+  float4 *n_ray_o, *n_ray_d, *n_thr;
+  uint64_t *n_rng;
+  uint32_t *n_meta, *n_slot;
+  /* by slot */
+  float4 *rad;      /* accumulated radiance, 4 hero wavelengths */
+  uint32_t *hero;   /* hero wavelength index (for the estimator) */
   uint32_t *gcnt;   /* guided renders: diffuse vertices recorded per path this pass (else NULL) */
 };
 
@@ -77,7 +91,6 @@ struct ShadeQueues {
 
 /* arguments of the persistent traversal kernel */
 struct TraceArgs {
-  const uint32_t *qin;         /* extend mode: ray queue */
   uint32_t *counts;            /* counter array (queue sizes, fetch counter) */
   uint32_t in_slot;            /* extend mode: slot of the ray-queue size */
   uint32_t n_direct;           /* primary mode: number of rays (paths 0..n-1) */
@@ -116,7 +129,7 @@ void launch_reset_counts(hipStream_t st, uint32_t *counts, uint32_t in_slot, uin
 void launch_set_count(hipStream_t st, uint32_t *counts, uint32_t slot, uint32_t value);
 /* zero n 64-bit words with a kernel (a memset node misbehaves in captured graphs) */
 void launch_zero_u64(hipStream_t st, uint64_t *p, uint32_t n);
-void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp, uint32_t *qout);
+void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp);
 /* persistent traversal: grid = resident blocks of the device; spill buffer of
  * trace_spill_words(blocks) words */
 uint32_t trace_grid_blocks();
@@ -149,8 +162,8 @@ void launch_generate_debug(hipStream_t st, const DevScene &sc, const PathBuf &pb
 void launch_primary_counting(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks);
 void launch_debug_color(hipStream_t st, const DevScene &sc, const PathBuf &pb, const DebugArgs &da,
                         const uint32_t *ray_counts, float *out);
-void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
-                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga);
+void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *counts,
+                  uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga);
 /* learned-proposal lobes for the Lambertian queue of the current bounce */
 void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GuideArgs &ga,
                         const uint32_t *queue, const uint32_t *count, uint32_t max_paths);

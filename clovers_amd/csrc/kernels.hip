@@ -88,15 +88,15 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *
   if (pred) q[base + (uint32_t)__popcll(below)] = item;
 }
 
-/* Block-aggregated append of `item` to queue `cls` (cls >= NC: no append).
- * Each wave counts its lanes per queue with a ballot and writes the counts to
- * LDS; NC threads turn them into per-wave offsets and do one global atomic
- * per queue for the whole block; then every lane stores its item.  Every
- * thread of the block must call this (it synchronises the block).  `lds`
- * holds NC * (BLOCK/64) + NC words. */
+/* Block-aggregated append to queue `cls` (cls >= NC: no append): the
+ * lane's position in the queue (~0u if none).  Each wave counts its lanes
+ * per queue with a ballot and writes the counts to LDS; NC threads turn them
+ * into per-wave offsets and do one global atomic per queue for the whole
+ * block.  Every thread of the block must call this (it synchronises the
+ * block).  `lds` holds NC * (BLOCK/64) + NC words.  block_append also stores
+ * `item` at that position. */
 template <int NC, int BLOCK>
-__device__ __forceinline__ void block_append(uint32_t cls, uint32_t item, uint32_t *const *queues, uint32_t *counters,
-                                             uint32_t *lds) {
+__device__ __forceinline__ uint32_t block_append_pos(uint32_t cls, uint32_t *counters, uint32_t *lds) {
   constexpr int WAVES = BLOCK / 64;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -119,8 +119,15 @@ __device__ __forceinline__ void block_append(uint32_t cls, uint32_t item, uint32
     lds[NC * WAVES + c] = sum ? atomicAdd(&counters[c], sum) : 0u;
   }
   __syncthreads();
-  if (cls < (uint32_t)NC) queues[cls][lds[NC * WAVES + cls] + lds[cls * WAVES + wave] + rank] = item;
+  const uint32_t pos = cls < (uint32_t)NC ? lds[NC * WAVES + cls] + lds[cls * WAVES + wave] + rank : 0xFFFFFFFFu;
   __syncthreads();
+  return pos;
+}
+template <int NC, int BLOCK>
+__device__ __forceinline__ void block_append(uint32_t cls, uint32_t item, uint32_t *const *queues, uint32_t *counters,
+                                             uint32_t *lds) {
+  const uint32_t pos = block_append_pos<NC, BLOCK>(cls, counters, lds);
+  if (cls < (uint32_t)NC) queues[cls][pos] = item;
 }
 
 /* -------------------------------------------------- primitive tests --- */
@@ -396,7 +403,7 @@ __device__ __forceinline__ uint32_t find_entry(const T *tab, uint32_t n, uint32_
 
 /* camera sample of sampler/random.rs:20-32 and Camera::get_ray (camera.rs:103-126);
  * path slot i belongs to the pass item whose range holds it */
-__global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenParams gp, uint32_t *qout) {
+__global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenParams gp) {
   uint32_t n = gp.paths;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const DevItem it = gp.items[find_entry(gp.items, gp.n_items, i, &DevItem::path_base)];
@@ -429,14 +436,16 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
             v * mk(c.vertical[0], c.vertical[1], c.vertical[2]) - org - off;
     dir = unit(dir);
     org = org + off;
+    /* slot i is entry i of the first bounce */
     pb.ray_o[i] = make_float4(org.x, org.y, org.z, __uint_as_float(jb.root));
     pb.ray_d[i] = make_float4(dir.x, dir.y, dir.z, time);
     pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    pb.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pb.rng[i] = s;
     pb.meta[i] = wl | (it.job << META_JOB_SHIFT); /* depth 0 */
+    pb.slot[i] = i;
+    pb.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pb.hero[i] = wl;
     if (pb.gcnt) pb.gcnt[i] = 0u;
-    qout[i] = i;
   }
 }
 
@@ -543,7 +552,7 @@ __global__ __launch_bounds__(LT_BLOCK) void k_trace_lds(DevScene sc, PathBuf pb,
     L.live = false;
     L.best = MISS_WORD;
     L.path = 0;
-    if (active) lane_start(L, pb, MODE == 0 ? ta.qin[i] : i);
+    if (active) lane_start(L, pb, i);
     else L.word = CHILD_EMPTY;
     while (L.word != CHILD_EMPTY) {
       if (!(L.word & CHILD_LEAF)) {
@@ -607,7 +616,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathB
     L.live = false;
     L.best = MISS_WORD;
     L.path = 0;
-    if (active) lane_start(L, pb, MODE == 0 ? ta.qin[i] : i);
+    if (active) lane_start(L, pb, i);
     else L.word = CHILD_EMPTY;
     const uint32_t v0 = nvisit, t0 = ntest;
     if (WW) {
@@ -675,7 +684,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathB
  * (while-while), then tests one leaf per lane.  Finished rays are written out
  * and, in the extend mode, appended to the shading queue of their material
  * class with wave-aggregated atomics.
- *   MODE_EXTEND:  rays are qin[0 .. counts[in_slot]) of the path state
+ *   MODE_EXTEND:  rays are entries 0 .. counts[in_slot] of the path state
  *   MODE_PRIMARY: rays are paths 0 .. n_direct, results go to ids/ts */
 #define MODE_EXTEND 0
 #define MODE_PRIMARY 1
@@ -709,7 +718,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, PathBuf pb, 
         if (base >= n) exhausted = true;
         if (!L.live) {
           uint32_t idx = base + (uint32_t)__popcll(idle & lt_mask);
-          if (idx < n) lane_start(L, pb, MODE == MODE_EXTEND ? ta.qin[idx] : idx);
+          if (idx < n) lane_start(L, pb, idx);
         }
       }
     }
@@ -999,22 +1008,32 @@ __device__ __forceinline__ void rotate_wl(uint32_t hero_idx, int wl[4]) {
 /* a radiance sample c0 (hero wavelength, already times the path throughput)
  * reached the path: credit it to the path's open training records, each as
  * c0 / (throughput right after that vertex) = radiance along its direction */
-__device__ __forceinline__ void gd_credit(const PathBuf &pb, const GuideArgs &ga, uint32_t i, float c0) {
+__device__ __forceinline__ void gd_credit(const PathBuf &pb, const GuideArgs &ga, uint32_t slot, float c0) {
   if (!(c0 > 0.0f)) return;
-  const uint32_t cnt = pb.gcnt[i];
+  const uint32_t cnt = pb.gcnt[slot];
   const uint32_t m = cnt < GD_SLOTS ? cnt : GD_SLOTS;
   for (uint32_t k = 0; k < m; k++) {
-    GuideRec &r = ga.rec[(size_t)i * GD_SLOTS + ((cnt - 1u - k) % GD_SLOTS)];
+    GuideRec &r = ga.rec[(size_t)slot * GD_SLOTS + ((cnt - 1u - k) % GD_SLOTS)];
     if (r.t0 > 0.0f) r.L += c0 / r.t0;
   }
 }
 
-/* One shading step of one path; returns true if the path continues.  EXT
- * adds what only some scenes need (checker textures, surface coordinates,
- * cone lights); batches without them use the leaner instantiation. */
+/* the state a surviving path carries into the next bounce */
+struct NextPath {
+  float4 o, d, thr;
+  uint64_t rng;
+  uint32_t meta, slot;
+};
+
+/* One shading step of the path at entry i; returns true if the path
+ * continues, with its next state in `nx`.  EXT adds what only some scenes
+ * need (checker textures, surface coordinates, cone lights); batches without
+ * them use the leaner instantiation. */
 template <uint32_t CLS, bool GUIDE, bool EXT>
-__device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint32_t i, const GuideArgs &ga) {
+__device__ __forceinline__ bool shade_path(const DevScene &sc, const PathBuf &pb, uint32_t i, const GuideArgs &ga,
+                                           NextPath &nx) {
   uint32_t meta = pb.meta[i];
+  const uint32_t slot = pb.slot[i];
   uint32_t hero_idx = meta & META_HERO_MASK;
   uint32_t depth = (meta >> META_DEPTH_SHIFT) & META_DEPTH_MASK;
   const DevJob &jb = sc.jobs[meta >> META_JOB_SHIFT];
@@ -1023,13 +1042,13 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
   float4 T = pb.thr[i];
   if (CLS == MC_MISS) {
     const float *bg = sc.spd + jb.bg_spd;
-    float4 L = pb.rad[i];
+    float4 L = pb.rad[slot];
     L.x += T.x * bg[wl[0] - CLV_LAMBDA_MIN];
     L.y += T.y * bg[wl[1] - CLV_LAMBDA_MIN];
     L.z += T.z * bg[wl[2] - CLV_LAMBDA_MIN];
     L.w += T.w * bg[wl[3] - CLV_LAMBDA_MIN];
-    pb.rad[i] = L;
-    if (GUIDE && ga.record && jb.guiding) gd_credit(pb, ga, i, T.x * bg[wl[0] - CLV_LAMBDA_MIN]);
+    pb.rad[slot] = L;
+    if (GUIDE && ga.record && jb.guiding) gd_credit(pb, ga, slot, T.x * bg[wl[0] - CLV_LAMBDA_MIN]);
     return false;
   }
   float4 o4 = pb.ray_o[i], d4 = pb.ray_d[i];
@@ -1045,13 +1064,13 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     }
     if (lit) {
       const float *e = sc.spd + (EXT ? tex_spd(m, m.emit_spd, h) : m.emit_spd);
-      float4 L = pb.rad[i];
+      float4 L = pb.rad[slot];
       L.x += T.x * e[wl[0] - CLV_LAMBDA_MIN];
       L.y += T.y * e[wl[1] - CLV_LAMBDA_MIN];
       L.z += T.z * e[wl[2] - CLV_LAMBDA_MIN];
       L.w += T.w * e[wl[3] - CLV_LAMBDA_MIN];
-      pb.rad[i] = L;
-      if (GUIDE && ga.record && jb.guiding) gd_credit(pb, ga, i, T.x * e[wl[0] - CLV_LAMBDA_MIN]);
+      pb.rad[slot] = L;
+      if (GUIDE && ga.record && jb.guiding) gd_credit(pb, ga, slot, T.x * e[wl[0] - CLV_LAMBDA_MIN]);
     }
     return false;
   }
@@ -1100,17 +1119,14 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     if (guided) cp = ga.alpha * gd_pdf(lob, ndn.x, ndn.y, ndn.z) + (1.0f - ga.alpha) * cp;
     float mis = 0.5f * lp + 0.5f * cp;
     float c2 = dot(h.n, ndn);
-    if (c2 < 0.0f) { /* scattering_pdf None: return emitted (0 for Lambertian) */
-      pb.rng[i] = s;
-      return false;
-    }
+    if (c2 < 0.0f) return false; /* scattering_pdf None: return emitted (0 for Lambertian) */
     float spdf = c2 / PI_F;
     T.x = T.x * att[0] * spdf / mis;
     T.y = T.y * att[1] * spdf / mis;
     T.z = T.z * att[2] * spdf / mis;
     T.w = T.w * att[3] * spdf / mis;
     if (GUIDE && ga.record && jb.guiding && depth + 1u <= jb.max_depth) {
-      const uint32_t cnt = pb.gcnt[i];
+      const uint32_t cnt = pb.gcnt[slot];
       GuideRec r;
       r.p[0] = 2.0f * (h.p.x - jb.lo[0]) / (jb.hi[0] - jb.lo[0]) - 1.0f;
       r.p[1] = 2.0f * (h.p.y - jb.lo[1]) / (jb.hi[1] - jb.lo[1]) - 1.0f;
@@ -1128,8 +1144,8 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
       r.L = 0.0f;
       r.stamp = ga.stamp;
       r.pad = 0u;
-      ga.rec[(size_t)i * GD_SLOTS + cnt % GD_SLOTS] = r;
-      pb.gcnt[i] = cnt + 1u;
+      ga.rec[(size_t)slot * GD_SLOTS + cnt % GD_SLOTS] = r;
+      pb.gcnt[slot] = cnt + 1u;
     }
   } else {
     if (CLS == MC_METAL) { /* metal.rs:26-52 */
@@ -1158,19 +1174,21 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, PathBuf &pb, uint
     T.z *= att[2];
     T.w *= att[3];
   }
-  pb.rng[i] = s;
   if (depth + 1u > jb.max_depth) return false; /* trace() at depth > max_depth returns 0 */
-  pb.thr[i] = T;
-  pb.ray_o[i] = make_float4(h.p.x, h.p.y, h.p.z, o4.w); /* .w: root node of the path's scene */
-  pb.ray_d[i] = make_float4(nd.x, nd.y, nd.z, d4.w); /* .w: the ray's instant stays with the path */
-  pb.meta[i] = (meta & ~(META_DEPTH_MASK << META_DEPTH_SHIFT)) | ((depth + 1u) << META_DEPTH_SHIFT);
+  nx.thr = T;
+  nx.o = make_float4(h.p.x, h.p.y, h.p.z, o4.w); /* .w: root node of the path's scene */
+  nx.d = make_float4(nd.x, nd.y, nd.z, d4.w);    /* .w: the ray's instant stays with the path */
+  nx.rng = s;
+  nx.meta = (meta & ~(META_DEPTH_MASK << META_DEPTH_SHIFT)) | ((depth + 1u) << META_DEPTH_SHIFT);
+  nx.slot = slot;
   return true;
 }
 
 template <bool GUIDE, bool EXT>
-__global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQueues sq, uint32_t *qout,
-                                                uint32_t *counts, uint32_t out_slot, GuideArgs ga) {
+__global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQueues sq, uint32_t *counts,
+                                                uint32_t out_slot, GuideArgs ga) {
   __shared__ uint32_t agg[1 * 4 + 1];
+  __shared__ NextPath stage[256]; /* survivors' state across the compaction (keeps it out of registers) */
   uint32_t c[MC_COUNT], chunks[MC_COUNT + 1];
   chunks[0] = 0;
 #pragma unroll
@@ -1180,11 +1198,9 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
   }
   const uint32_t total = chunks[MC_COUNT];
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t *const out_q[1] = {qout};
   /* block-uniform loop: each iteration the block's 4 waves take 4 consecutive 64-path chunks */
   for (uint32_t first = blockIdx.x * 4u; first < total; first += gridDim.x * 4u) {
     const uint32_t wv = first + (threadIdx.x >> 6);
-    uint32_t path = 0;
     bool cont = false;
     if (wv < total) {
       uint32_t cls = 0;
@@ -1194,17 +1210,28 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
       cls = __builtin_amdgcn_readfirstlane(cls);
       uint32_t idx = (wv - chunks[cls]) * 64u + lane;
       if (idx < c[cls]) {
-        path = sq.q[cls][idx];
+        const uint32_t path = sq.q[cls][idx];
+        NextPath &nx = stage[threadIdx.x];
         switch (cls) {
-          case MC_MISS: cont = shade_path<MC_MISS, GUIDE, EXT>(sc, pb, path, ga); break;
-          case MC_LIGHT: cont = shade_path<MC_LIGHT, GUIDE, EXT>(sc, pb, path, ga); break;
-          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN, GUIDE, EXT>(sc, pb, path, ga); break;
-          case MC_METAL: cont = shade_path<MC_METAL, GUIDE, EXT>(sc, pb, path, ga); break;
-          default: cont = shade_path<MC_GLASS, GUIDE, EXT>(sc, pb, path, ga); break;
+          case MC_MISS: cont = shade_path<MC_MISS, GUIDE, EXT>(sc, pb, path, ga, nx); break;
+          case MC_LIGHT: cont = shade_path<MC_LIGHT, GUIDE, EXT>(sc, pb, path, ga, nx); break;
+          case MC_LAMBERTIAN: cont = shade_path<MC_LAMBERTIAN, GUIDE, EXT>(sc, pb, path, ga, nx); break;
+          case MC_METAL: cont = shade_path<MC_METAL, GUIDE, EXT>(sc, pb, path, ga, nx); break;
+          default: cont = shade_path<MC_GLASS, GUIDE, EXT>(sc, pb, path, ga, nx); break;
         }
       }
     }
-    block_append<1, 256>(cont ? 0u : 1u, path, out_q, counts + out_slot, agg);
+    /* survivors are compacted into the next bounce's entries, in block order */
+    const uint32_t e = block_append_pos<1, 256>(cont ? 0u : 1u, counts + out_slot, agg);
+    if (cont) {
+      const NextPath &nx = stage[threadIdx.x];
+      pb.n_ray_o[e] = nx.o;
+      pb.n_ray_d[e] = nx.d;
+      pb.n_thr[e] = nx.thr;
+      pb.n_rng[e] = nx.rng;
+      pb.n_meta[e] = nx.meta;
+      pb.n_slot[e] = nx.slot;
+    }
   }
 }
 
@@ -1390,7 +1417,7 @@ __global__ __launch_bounds__(256) void k_resolve(PathBuf pb, ResolveParams rp, c
     for (uint32_t j = 0; j < it.samples; j++) {
       uint32_t i = it.path_base + j * it.pixels + o;
       float4 L = pb.rad[i];
-      uint32_t hero = pb.meta[i] & META_HERO_MASK;
+      uint32_t hero = pb.hero[i];
       float c[3] = {0.0f, 0.0f, 0.0f};
       if (L.y == 0.0f && L.z == 0.0f && L.w == 0.0f) {
         float4 x = cie[hero];
@@ -1458,8 +1485,8 @@ void launch_zero_u64(hipStream_t st, uint64_t *p, uint32_t n) {
 void launch_set_count(hipStream_t st, uint32_t *counts, uint32_t slot, uint32_t value) {
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, st, counts, slot, value);
 }
-void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp, uint32_t *qout) {
-  hipLaunchKernelGGL(k_generate, dim3(grid_for(gp.paths, 256, 65535)), dim3(256), 0, st, sc, pb, gp, qout);
+void launch_generate(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GenParams &gp) {
+  hipLaunchKernelGGL(k_generate, dim3(grid_for(gp.paths, 256, 65535)), dim3(256), 0, st, sc, pb, gp);
 }
 static uint32_t g_trace_blocks = 0;
 
@@ -1544,17 +1571,17 @@ void launch_debug_color(hipStream_t st, const DevScene &sc, const PathBuf &pb, c
   hipLaunchKernelGGL(k_debug_color, dim3(grid_for(da.width * da.height, 256, 8192)), dim3(256), 0, st, sc, pb, da,
                      ray_counts, out);
 }
-void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *qout,
-                  uint32_t *counts, uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga) {
+void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const ShadeQueues &sq, uint32_t *counts,
+                  uint32_t out_slot, uint32_t max_paths, uint32_t grid_cap, const GuideArgs &ga) {
   uint32_t g = grid_for(max_paths + 64u * MC_COUNT, 256, grid_cap);
   if (ga.on && sc.has_ext)
-    hipLaunchKernelGGL((k_shade<true, true>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+    hipLaunchKernelGGL((k_shade<true, true>), dim3(g), dim3(256), 0, st, sc, pb, sq, counts, out_slot, ga);
   else if (ga.on)
-    hipLaunchKernelGGL((k_shade<true, false>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(g), dim3(256), 0, st, sc, pb, sq, counts, out_slot, ga);
   else if (sc.has_ext)
-    hipLaunchKernelGGL((k_shade<false, true>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+    hipLaunchKernelGGL((k_shade<false, true>), dim3(g), dim3(256), 0, st, sc, pb, sq, counts, out_slot, ga);
   else
-    hipLaunchKernelGGL((k_shade<false, false>), dim3(g), dim3(256), 0, st, sc, pb, sq, qout, counts, out_slot, ga);
+    hipLaunchKernelGGL((k_shade<false, false>), dim3(g), dim3(256), 0, st, sc, pb, sq, counts, out_slot, ga);
 }
 void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GuideArgs &ga,
                         const uint32_t *queue, const uint32_t *count, uint32_t max_paths) {
