@@ -89,7 +89,7 @@ class Tuning(C.Structure):
     _fields_ = [("target_paths", C.c_uint32), ("trace_kernel", C.c_uint32), ("lds_bytes", C.c_uint32),
                 ("shade_grid_cap", C.c_uint32), ("guide_train_spp", C.c_uint32), ("guide_alpha", C.c_float),
                 ("guide_lr", C.c_float), ("guide_records", C.c_uint32), ("guide_wclip", C.c_float),
-                ("guide_rounds", C.c_uint32), ("use_graph", C.c_uint32), ("_pad", C.c_uint32)]
+                ("guide_rounds", C.c_uint32), ("use_graph", C.c_uint32), ("guide_credit", C.c_uint32)]
 
 
 class PassItem(C.Structure):

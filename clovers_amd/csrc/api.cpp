@@ -64,6 +64,8 @@ struct SceneRes {
 };
 
 /* the guiding network and its training buffers (allocated on first use) */
+#define GUIDE_STEP 16384u /* default records per Adam step */
+
 struct GuideState {
   float *P = nullptr, *M = nullptr, *V = nullptr;  /* fp32 parameters, Adam moments */
   _Float16 *Wh = nullptr;                          /* fp16 operand copies */
@@ -72,6 +74,7 @@ struct GuideState {
   uint32_t cap_paths = 0;                          /* path capacity of the buffers below */
   float *lobes = nullptr;
   GuideRec *rec = nullptr;
+  float *recL = nullptr;     /* radiance credited to each record this pass */
   uint32_t *gcnt = nullptr;
   uint32_t *list = nullptr;
   uint32_t *bcount = nullptr;
@@ -81,14 +84,15 @@ struct GuideState {
   uint32_t stamp = 0;                              /* last pass id */
   bool init = false, trained = false;
   /* training schedule (clv_tuning) */
-  uint32_t train_spp = 4;
+  uint32_t train_spp = 64;
   float alpha = 0.5f;
   float lr = 0.005f;
-  uint32_t records = 262144;
+  uint32_t records = 2097152;
   float wclip = 1e6f;
-  uint32_t rounds = 16;
+  uint32_t rounds = 4;
+  uint32_t credit = 1;       /* clv_tuning.guide_credit */
+  uint32_t step = GUIDE_STEP; /* records per Adam step */
 };
-#define GUIDE_STEP 16384u
 
 }  // namespace
 
@@ -426,24 +430,26 @@ clv_status guide_ensure_partial(clv_ctx *c, uint32_t waves) {
 
 void guide_free(GuideState &g) {
   dfree(g.P); dfree(g.M); dfree(g.V); dfree(g.Wh); dfree(g.partial); dfree(g.totals);
-  dfree(g.lobes); dfree(g.rec); dfree(g.gcnt); dfree(g.list); dfree(g.bcount); dfree(g.bweight);
+  dfree(g.lobes); dfree(g.rec); dfree(g.recL); dfree(g.gcnt); dfree(g.list); dfree(g.bcount); dfree(g.bweight);
 }
 
 clv_status guide_ensure_paths(clv_ctx *c, uint32_t paths) {
   GuideState &g = c->guide;
   if (paths <= g.cap_paths) return CLV_OK;
-  dfree(g.lobes); dfree(g.rec); dfree(g.gcnt); dfree(g.list); dfree(g.bcount); dfree(g.bweight);
-  g.lobes = nullptr; g.rec = nullptr; g.gcnt = nullptr; g.list = nullptr; g.bcount = nullptr; g.bweight = nullptr;
+  dfree(g.lobes); dfree(g.rec); dfree(g.recL); dfree(g.gcnt); dfree(g.list); dfree(g.bcount); dfree(g.bweight);
+  g.lobes = nullptr; g.rec = nullptr; g.recL = nullptr; g.gcnt = nullptr; g.list = nullptr; g.bcount = nullptr;
+  g.bweight = nullptr;
   g.cap_paths = 0;
   const size_t slots = (size_t)paths * GD_SLOTS;
   HIPCHK(c, dalloc(&g.lobes, (size_t)paths * GD_LFLOATS));
   HIPCHK(c, dalloc(&g.rec, slots));
+  HIPCHK(c, dalloc(&g.recL, slots));
   HIPCHK(c, dalloc(&g.gcnt, paths));
   HIPCHK(c, dalloc(&g.list, slots));
   HIPCHK(c, dalloc(&g.bcount, slots / 256 + 1));
   HIPCHK(c, dalloc(&g.bweight, slots / 256 + 1));
-  /* stale stamps must never match a future pass id */
   HIPCHK(c, hipMemsetAsync(g.rec, 0, slots * sizeof(GuideRec), c->stream));
+  HIPCHK(c, hipMemsetAsync(g.recL, 0, slots * sizeof(float), c->stream));
   g.cap_paths = paths;
   return CLV_OK;
 }
@@ -485,12 +491,12 @@ clv_status guide_reset(clv_ctx *c, uint64_t seed) {
 void guide_train_round(clv_ctx *c, uint32_t paths, uint32_t stamp, float *grad_out, float wclip) {
   GuideState &g = c->guide;
   const uint32_t cap = grad_out ? paths * GD_SLOTS : g.records;
-  launch_guide_collect(c->stream, g.rec, paths * GD_SLOTS, stamp, g.bcount, g.bweight, g.totals, g.list, cap,
+  launch_guide_collect(c->stream, g.rec, g.recL, paths * GD_SLOTS, g.bcount, g.bweight, g.totals, g.list, cap,
                        (unsigned long long *)c->stats + 4);
-  const uint32_t step = grad_out ? cap : GUIDE_STEP;
+  const uint32_t step = grad_out ? cap : g.step;
   for (uint32_t b = 0; b < cap; b += step) {
     if (!grad_out) g.t++;
-    launch_guide_step(c->stream, g.rec, g.list, g.totals, cap, b, std::min(step, cap - b), wclip, g.P, g.M, g.V,
+    launch_guide_step(c->stream, g.rec, g.recL, g.list, g.totals, cap, b, std::min(step, cap - b), wclip, g.P, g.M, g.V,
                       g.Wh, g.partial, g.lr, std::max<uint32_t>(g.t, 1), grad_out);
   }
 }
@@ -541,7 +547,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   const bool frozen = (flags & CLV_FLAG_GUIDE_FROZEN) != 0;
   if (any_guide) {
     CHK(guide_ensure_paths(c, pl.max_paths));
-    CHK(guide_ensure_partial(c, GUIDE_STEP / 64u));
+    CHK(guide_ensure_partial(c, gs.step / 64u));
     if (!(flags & CLV_FLAG_GUIDE_KEEP) || !gs.init) CHK(guide_reset(c, guide_seed));
   }
   /* the LDS window caches the scene with the most samples */
@@ -580,6 +586,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   size_t ev = 2;
   PathBuf pb = c->pb;
   pb.gcnt = any_guide ? gs.gcnt : nullptr;
+  pb.recL = any_guide ? gs.recL : nullptr;
   GuideArgs ga;
   memset(&ga, 0, sizeof(ga));
   if (any_guide) {
@@ -588,6 +595,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     ga.Wh = gs.Wh;
     ga.P = gs.P;
     ga.on = 1u;
+    ga.indirect = gs.credit;
     ga.stats = (unsigned long long *)c->stats;
   }
   /* timing events of the guide's launches: [start, end) pairs */
@@ -679,8 +687,8 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     add(pl.res.data(), pl.res.size() * 4);
     add(pl.depth.data(), pl.depth.size() * 4);
     add(pl.train.data(), pl.train.size());
-    const uint32_t sc[9] = {flags, any_sq, any_guide, frozen, (uint32_t)c->trace_variant, c->grid_cap, ds.has_xforms,
-                            gs.train_spp, ds.has_ext};
+    const uint32_t sc[10] = {flags, any_sq, any_guide, frozen, (uint32_t)c->trace_variant, c->grid_cap, ds.has_xforms,
+                             gs.train_spp, ds.has_ext, gs.credit};
     add(sc, sizeof(sc));
     const float gf[4] = {gs.alpha, gs.lr, gs.wclip, (float)gs.records};
     add(gf, sizeof(gf));
@@ -825,6 +833,7 @@ clv_status clv_create(int hip_device, clv_ctx **out) {
   }
   c->trace_blocks = trace_grid_blocks();
   if (const char *e = getenv("CLV_TRACE_BLOCKS")) c->trace_blocks = (uint32_t)strtoul(e, nullptr, 10);
+  if (const char *e = getenv("CLV_GUIDE_STEP")) c->guide.step = std::max<uint32_t>(64u, (uint32_t)strtoul(e, nullptr, 10) & ~63u);
   if (const char *e = getenv("CLV_TRACE_KERNEL"))
     c->trace_variant = strcmp(e, "persistent") == 0 ? TRACE_PERSISTENT
                        : strcmp(e, "ww") == 0       ? TRACE_WW
@@ -1075,6 +1084,7 @@ clv_status clv_tuning_get(const clv_ctx *c, clv_tuning *out) {
   out->guide_wclip = c->guide.wclip;
   out->guide_rounds = c->guide.rounds;
   out->use_graph = c->use_graph ? 1u : 0u;
+  out->guide_credit = c->guide.credit;
   return CLV_OK;
 }
 
@@ -1090,6 +1100,7 @@ clv_status clv_tuning_set(clv_ctx *c, const clv_tuning *t) {
   if (!(t->guide_lr > 0.0f && t->guide_lr < 1.0f)) return fail(c, CLV_EINVAL, "guide_lr outside (0, 1)");
   if (t->guide_records < 64 || t->guide_records > (1u << 24)) return fail(c, CLV_EINVAL, "guide_records outside [64, 2^24]");
   if (!(t->guide_wclip >= 1.0f)) return fail(c, CLV_EINVAL, "guide_wclip below 1");
+  if (t->guide_credit > 1u) return fail(c, CLV_EINVAL, "guide_credit is 0 or 1");
   c->target_paths = t->target_paths;
   c->trace_variant = (int)t->trace_kernel;
   c->lds_block_bytes = t->lds_bytes;
@@ -1101,6 +1112,7 @@ clv_status clv_tuning_set(clv_ctx *c, const clv_tuning *t) {
   c->guide.wclip = t->guide_wclip;
   c->guide.rounds = t->guide_rounds;
   c->use_graph = t->use_graph != 0;
+  c->guide.credit = t->guide_credit;
   return CLV_OK;
 }
 
@@ -1173,6 +1185,7 @@ clv_status clv_guide_grad(clv_ctx *c, const float *samples, uint32_t n, float *g
   CHK(guide_ensure_partial(c, (n + 63) / 64));
   const uint32_t stamp = ++g.stamp;
   std::vector<GuideRec> recs(n);
+  std::vector<float> recL(n);
   for (uint32_t i = 0; i < n; i++) {
     const float *sm = samples + 11 * (size_t)i;
     GuideRec &r = recs[i];
@@ -1186,15 +1199,16 @@ clv_status clv_guide_grad(clv_ctx *c, const float *samples, uint32_t n, float *g
     r.pdf = 1.0f;
     r.cosv = 1.0f;
     r.t0 = 1.0f;
-    r.L = sm[10];
+    recL[i] = sm[10];
     r.stamp = stamp;
   }
   float *dgrad = nullptr;
   HIPCHK(c, dalloc(&dgrad, GD_NPARAM));
   hipError_t e = hipMemcpyAsync(g.rec, recs.data(), n * sizeof(GuideRec), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(g.recL, recL.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
-    launch_guide_collect(c->stream, g.rec, n, stamp, g.bcount, g.bweight, g.totals, g.list, n, nullptr);
-    launch_guide_step(c->stream, g.rec, g.list, g.totals, n, 0, n, FLT_MAX, g.P, g.M, g.V, g.Wh, g.partial, g.lr, 1,
+    launch_guide_collect(c->stream, g.rec, g.recL, n, g.bcount, g.bweight, g.totals, g.list, n, nullptr);
+    launch_guide_step(c->stream, g.rec, g.recL, g.list, g.totals, n, 0, n, FLT_MAX, g.P, g.M, g.V, g.Wh, g.partial, g.lr, 1,
                       dgrad);
     e = hipGetLastError();
   }

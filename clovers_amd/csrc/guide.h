@@ -71,7 +71,7 @@ struct GuideRec {
   float pdf;        /* density the direction was drawn with */
   float cosv;       /* cos(theta) of the direction */
   float t0;         /* hero throughput right after this vertex */
-  float L;          /* hero radiance that arrived along d afterwards (sum of c / t0) */
+  uint32_t pad0;    /* (the radiance that arrives afterwards is kept apart, in recL) */
   uint32_t stamp;   /* pass that wrote it */
   uint32_t pad;
 };                  /* 64 bytes */

@@ -27,23 +27,33 @@ namespace clv {
 #define GT_LD 72     /* LDS row stride (halfs) of activation tiles: 64 + 8 against bank conflicts */
 #define GT_OG 49     /* LDS row stride (floats) of output / gradient rows */
 
-__device__ __forceinline__ bool rec_valid(const GuideRec &r, uint32_t stamp) {
-  return r.stamp == stamp && r.L > 0.0f && isfinite(r.L) && r.pdf > 0.0f && isfinite(r.pdf) && r.t0 > 0.0f;
+/* a record of this pass with radiance (recL is zeroed when the pass starts
+ * and when a record is written); the record itself is read only then */
+__device__ __forceinline__ bool rec_valid(const GuideRec *rec, const float *recL, uint32_t i, float *w) {
+  const float L = recL[i];
+  if (!(L > 0.0f && isfinite(L))) return false;
+  const GuideRec &r = rec[i];
+  if (!(r.pdf > 0.0f && isfinite(r.pdf) && r.t0 > 0.0f)) return false;
+  *w = L * r.cosv / r.pdf;
+  return true;
 }
 
 /* ---------------------------------------------------------- compaction --- */
-__global__ __launch_bounds__(256) void k_guide_count(const GuideRec *rec, uint32_t n, uint32_t stamp,
+/* each block covers GC_PER consecutive records, 256 at a time, so the scan
+ * has few block counts to add up */
+#define GC_PER 4096u
+__global__ __launch_bounds__(256) void k_guide_count(const GuideRec *rec, const float *recL, uint32_t n,
                                                      uint32_t *bcount, float *bweight) {
   __shared__ uint32_t sc[256];
   __shared__ float sw[256];
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   uint32_t c = 0;
   float w = 0.0f;
-  if (i < n) {
-    const GuideRec r = rec[i];
-    if (rec_valid(r, stamp)) {
-      c = 1;
-      w = r.L * r.cosv / r.pdf;
+  for (uint32_t k = 0; k < GC_PER; k += 256u) {
+    const uint32_t i = blockIdx.x * GC_PER + k + threadIdx.x;
+    float wt;
+    if (i < n && rec_valid(rec, recL, i, &wt)) {
+      c++;
+      w += wt;
     }
   }
   sc[threadIdx.x] = c;
@@ -102,18 +112,24 @@ __global__ __launch_bounds__(1024) void k_guide_scan(uint32_t *bcount, const flo
   }
 }
 
-__global__ __launch_bounds__(256) void k_guide_scatter(const GuideRec *rec, uint32_t n, uint32_t stamp,
+__global__ __launch_bounds__(256) void k_guide_scatter(const GuideRec *rec, const float *recL, uint32_t n,
                                                        const uint32_t *boffset, uint32_t *list) {
   __shared__ uint32_t wcount[4];
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  const bool v = i < n && rec_valid(rec[i], stamp);
-  const uint64_t m = __ballot(v);
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  if (lane == 0) wcount[w] = (uint32_t)__popcll(m);
-  __syncthreads();
   uint32_t off = boffset[blockIdx.x];
-  for (uint32_t k = 0; k < w; k++) off += wcount[k];
-  if (v) list[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+  for (uint32_t k = 0; k < GC_PER; k += 256u) {  /* block-uniform: chunks in record order */
+    const uint32_t i = blockIdx.x * GC_PER + k + threadIdx.x;
+    float wt;
+    const bool v = i < n && rec_valid(rec, recL, i, &wt);
+    const uint64_t m = __ballot(v);
+    if (lane == 0) wcount[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t o = off;
+    for (uint32_t q = 0; q < w; q++) o += wcount[q];
+    if (v) list[o + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    off += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
+  }
 }
 
 /* ------------------------------------------------------------ training --- */
@@ -168,6 +184,7 @@ __device__ __forceinline__ void gd_gemm_ab(const _Float16 *A, int lda, const gh4
 
 struct GuideTrainArgs {
   const GuideRec *rec;
+  const float *recL;
   const uint32_t *list;    /* compacted record slots */
   const uint32_t *totals;  /* [0] valid records, [1] mean weight (float bits) */
   uint32_t cap;            /* records used per round */
@@ -196,7 +213,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideTrainArgs a) {
     if (u < nuse && wmean > 0.0f) {
       const uint32_t slot = a.list[(uint32_t)(((uint64_t)u * nvalid) / nuse)];
       const GuideRec R = a.rec[slot];
-      wgt = fminf(R.L * R.cosv / R.pdf / wmean, a.wclip);
+      wgt = fminf(a.recL[slot] * R.cosv / R.pdf / wmean, a.wclip);
       for (int k = 0; k < 3; k++) {
         in[k] = R.p[k];
         in[3 + k] = R.n[k];
@@ -428,19 +445,21 @@ __global__ __launch_bounds__(64) void k_guide_eval(const float *in, uint32_t n, 
 }
 
 /* ---------------------------------------------------------- launchers --- */
-void launch_guide_collect(hipStream_t st, const GuideRec *rec, uint32_t n, uint32_t stamp, uint32_t *bcount,
+void launch_guide_collect(hipStream_t st, const GuideRec *rec, const float *recL, uint32_t n, uint32_t *bcount,
                           float *bweight, uint32_t *totals, uint32_t *list, uint32_t cap, unsigned long long *used) {
-  const uint32_t nb = (n + 255u) / 256u;
-  hipLaunchKernelGGL(k_guide_count, dim3(nb), dim3(256), 0, st, rec, n, stamp, bcount, bweight);
+  const uint32_t nb = (n + GC_PER - 1u) / GC_PER;
+  hipLaunchKernelGGL(k_guide_count, dim3(nb), dim3(256), 0, st, rec, recL, n, bcount, bweight);
   hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(1024), 0, st, bcount, bweight, nb, totals, cap, used);
-  hipLaunchKernelGGL(k_guide_scatter, dim3(nb), dim3(256), 0, st, rec, n, stamp, bcount, list);
+  hipLaunchKernelGGL(k_guide_scatter, dim3(nb), dim3(256), 0, st, rec, recL, n, bcount, list);
 }
 
-void launch_guide_step(hipStream_t st, const GuideRec *rec, const uint32_t *list, const uint32_t *totals,
-                       uint32_t cap, uint32_t step_base, uint32_t step_len, float wclip, float *P, float *m,
-                       float *v, _Float16 *Wh, float *partial, float lr, uint32_t t, float *grad_out) {
+void launch_guide_step(hipStream_t st, const GuideRec *rec, const float *recL, const uint32_t *list,
+                       const uint32_t *totals, uint32_t cap, uint32_t step_base, uint32_t step_len, float wclip,
+                       float *P, float *m, float *v, _Float16 *Wh, float *partial, float lr, uint32_t t,
+                       float *grad_out) {
   GuideTrainArgs ta;
   ta.rec = rec;
+  ta.recL = recL;
   ta.list = list;
   ta.totals = totals;
   ta.cap = cap;

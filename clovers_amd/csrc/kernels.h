@@ -69,6 +69,9 @@ struct PathBuf {
   float4 *rad;      /* accumulated radiance, 4 hero wavelengths */
   uint32_t *hero;   /* hero wavelength index (for the estimator) */
   uint32_t *gcnt;   /* guided renders: diffuse vertices recorded per path this pass (else NULL) */
+  float *recL;      /* guided renders, by slot x GD_SLOTS: hero radiance that arrived along each
+                       training record's direction afterwards (sum of c / t0); zeroed per pass, so
+                       the training round scans 4 bytes per record */
 };
 
 /* guided sampling state of a pass */
@@ -81,6 +84,9 @@ struct GuideArgs {
   uint32_t stamp;          /* pass id written into records */
   uint32_t record;         /* write training records */
   uint32_t on;             /* the batch has guided jobs: lobes/rec/gcnt are valid */
+  uint32_t indirect;       /* 1: a record is not credited with emission reached straight from it
+                              (direct light, which the light-tree half samples), only through a
+                              specular chain or further bounces */
   unsigned long long *stats; /* stats[3] counts network evaluations */
 };
 
@@ -168,11 +174,12 @@ void launch_shade(hipStream_t st, const DevScene &sc, const PathBuf &pb, const S
 void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, const GuideArgs &ga,
                         const uint32_t *queue, const uint32_t *count, uint32_t max_paths);
 /* guide.hip: record compaction, training steps, network probes */
-void launch_guide_collect(hipStream_t st, const GuideRec *rec, uint32_t n, uint32_t stamp, uint32_t *bcount,
+void launch_guide_collect(hipStream_t st, const GuideRec *rec, const float *recL, uint32_t n, uint32_t *bcount,
                           float *bweight, uint32_t *totals, uint32_t *list, uint32_t cap, unsigned long long *used);
-void launch_guide_step(hipStream_t st, const GuideRec *rec, const uint32_t *list, const uint32_t *totals,
-                       uint32_t cap, uint32_t step_base, uint32_t step_len, float wclip, float *P, float *m,
-                       float *v, _Float16 *Wh, float *partial, float lr, uint32_t t, float *grad_out);
+void launch_guide_step(hipStream_t st, const GuideRec *rec, const float *recL, const uint32_t *list,
+                       const uint32_t *totals, uint32_t cap, uint32_t step_base, uint32_t step_len, float wclip,
+                       float *P, float *m, float *v, _Float16 *Wh, float *partial, float lr, uint32_t t,
+                       float *grad_out);
 void launch_guide_halves(hipStream_t st, const float *P, _Float16 *Wh);
 void launch_guide_eval(hipStream_t st, const float *in, uint32_t n, const _Float16 *Wh, const float *P, float *raw,
                        float *lobes);

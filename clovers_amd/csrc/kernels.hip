@@ -445,7 +445,11 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
     pb.slot[i] = i;
     pb.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pb.hero[i] = wl;
-    if (pb.gcnt) pb.gcnt[i] = 0u;
+    if (pb.gcnt) {
+      pb.gcnt[i] = 0u;
+#pragma unroll
+      for (uint32_t k = 0; k < GD_SLOTS; k++) pb.recL[(size_t)i * GD_SLOTS + k] = 0.0f;
+    }
   }
 }
 
@@ -1008,13 +1012,15 @@ __device__ __forceinline__ void rotate_wl(uint32_t hero_idx, int wl[4]) {
 /* a radiance sample c0 (hero wavelength, already times the path throughput)
  * reached the path: credit it to the path's open training records, each as
  * c0 / (throughput right after that vertex) = radiance along its direction */
+#define GD_SPECULAR 0x80000000u /* gcnt flag: a specular vertex since the latest record */
 __device__ __forceinline__ void gd_credit(const PathBuf &pb, const GuideArgs &ga, uint32_t slot, float c0) {
   if (!(c0 > 0.0f)) return;
-  const uint32_t cnt = pb.gcnt[slot];
+  const uint32_t g = pb.gcnt[slot], cnt = g & ~GD_SPECULAR;
   const uint32_t m = cnt < GD_SLOTS ? cnt : GD_SLOTS;
-  for (uint32_t k = 0; k < m; k++) {
-    GuideRec &r = ga.rec[(size_t)slot * GD_SLOTS + ((cnt - 1u - k) % GD_SLOTS)];
-    if (r.t0 > 0.0f) r.L += c0 / r.t0;
+  for (uint32_t k = (ga.indirect && !(g & GD_SPECULAR)) ? 1u : 0u; k < m; k++) {
+    const size_t j = (size_t)slot * GD_SLOTS + ((cnt - 1u - k) % GD_SLOTS);
+    const float t0 = ga.rec[j].t0;
+    if (t0 > 0.0f) pb.recL[j] += c0 / t0;
   }
 }
 
@@ -1126,7 +1132,7 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, const PathBuf &pb
     T.z = T.z * att[2] * spdf / mis;
     T.w = T.w * att[3] * spdf / mis;
     if (GUIDE && ga.record && jb.guiding && depth + 1u <= jb.max_depth) {
-      const uint32_t cnt = pb.gcnt[slot];
+      const uint32_t cnt = pb.gcnt[slot] & ~GD_SPECULAR;
       GuideRec r;
       r.p[0] = 2.0f * (h.p.x - jb.lo[0]) / (jb.hi[0] - jb.lo[0]) - 1.0f;
       r.p[1] = 2.0f * (h.p.y - jb.lo[1]) / (jb.hi[1] - jb.lo[1]) - 1.0f;
@@ -1141,13 +1147,18 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, const PathBuf &pb
       r.pdf = mis;
       r.cosv = c2;
       r.t0 = T.x;
-      r.L = 0.0f;
+      r.pad0 = 0u;
       r.stamp = ga.stamp;
       r.pad = 0u;
       ga.rec[(size_t)slot * GD_SLOTS + cnt % GD_SLOTS] = r;
+      pb.recL[(size_t)slot * GD_SLOTS + cnt % GD_SLOTS] = 0.0f;
       pb.gcnt[slot] = cnt + 1u;
     }
   } else {
+    if (GUIDE && ga.record && ga.indirect && jb.guiding) {
+      const uint32_t g = pb.gcnt[slot];
+      if (g && !(g & GD_SPECULAR)) pb.gcnt[slot] = g | GD_SPECULAR;
+    }
     if (CLS == MC_METAL) { /* metal.rs:26-52 */
       V r = reflect(d, h.n);
       nd = unit(r + m.fuzz * random_unit_vector(s));

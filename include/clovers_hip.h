@@ -322,18 +322,20 @@ typedef struct {
   uint32_t trace_kernel;    /* clv_trace_kernel */
   uint32_t lds_bytes;       /* dynamic LDS per block of CLV_TRACE_LDS_CACHED (<= 64512) */
   uint32_t shade_grid_cap;  /* largest grid of the shading kernel, in 256-thread blocks */
-  uint32_t guide_train_spp; /* guided jobs: samples per pixel between training rounds (default 4) */
+  uint32_t guide_train_spp; /* guided jobs: samples per pixel between training rounds (default 64) */
   float guide_alpha;        /* at a diffuse vertex, share of the cosine half of the mixture drawn from the
                                learned lobes (default 0.5) */
   float guide_lr;           /* Adam learning rate (default 0.005) */
-  uint32_t guide_records;   /* training records used per round (default 262144, in steps of 16384) */
+  uint32_t guide_records;   /* training records used per round (default 2^21, in steps of 16384) */
   float guide_wclip;        /* training weight F/p capped at this multiple of its mean (default 1e6) */
-  uint32_t guide_rounds;    /* training rounds per guided render (default 16); later samples use the
+  uint32_t guide_rounds;    /* training rounds per guided render (default 4); later samples use the
                                trained network in large passes without training */
   uint32_t use_graph;       /* 1 (default): a render repeated with identical arguments is captured as a hipGraph
                                on its second call and replayed from then on; never for CLV_FLAG_TIME_KERNELS
                                or guided CLV_FLAG_GUIDE_KEEP */
-  uint32_t _pad;
+  uint32_t guide_credit;    /* radiance a training record is credited with: 1 (default) only what arrives
+                               through a specular chain or further bounces (direct light is the light-tree
+                               half's to sample); 0 everything that reaches the path after the vertex */
 } clv_tuning;
 clv_status clv_tuning_get(const clv_ctx *ctx, clv_tuning *out);
 clv_status clv_tuning_set(clv_ctx *ctx, const clv_tuning *t);
