@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--spp", type=int, default=0,
                     help="override the workload's samples per pixel (profiling runs: per-launch figures do not "
                          "depend on it; the JSON line then names the reduced workload)")
+    ap.add_argument("--builder", default="host", choices=["host", "device"],
+                    help="scene BVH builder: host binned SAH or the GPU clustering build")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="before rendering, copy 1 GiB device-to-device 3 times (a plain copy kernel of known "
                          "size, so profiler byte counters can be scaled)")
@@ -261,9 +263,9 @@ def main():
         t0 = time.perf_counter()
         sids, build_ms = [], 0.0
         for sc, _, _, _ in batch:
-            sids.append(r.upload(sc))
+            sids.append(r.upload(sc, builder=args.builder))
             build_ms += r.counters()["build_ms"]
-        log("uploaded in %.1f s (host builds %.1f s)" % (time.perf_counter() - t0, build_ms / 1e3))
+        log("uploaded in %.1f s (%s builds %.1f s)" % (time.perf_counter() - t0, args.builder, build_ms / 1e3))
 
         def jobs_for(flags=0, count=False):
             return [(sid, abi.render_opts(side, side, max(1, spp // 16) if count else spp, w["max_depth"],
@@ -283,9 +285,9 @@ def main():
         scene = getattr(synth, w["scene"])()
         log("workload", args.workload, w)
         t0 = time.perf_counter()
-        sid = r.upload(scene)
-        log("scene upload %.1f ms (host build %.1f ms)" % ((time.perf_counter() - t0) * 1e3,
-                                                          r.counters()["build_ms"]))
+        sid = r.upload(scene, builder=args.builder)
+        build_ms = r.counters()["build_ms"]
+        log("scene upload %.1f ms (%s build %.1f ms)" % ((time.perf_counter() - t0) * 1e3, args.builder, build_ms))
 
         def jobs_for(flags=0, count=False):
             spp = w["count_spp"] if count else w["samples"]
@@ -384,6 +386,7 @@ def main():
         "config": dict({"workload": wl_name}, **wcfg, **{
                    "samples_per_step": samples, "nonfinite_pixels": nonfinite, "device_ms_per_step": round(float(np.mean(gpu_ms)), 3),
                    "parallelism": "1 GPU, wavefront",
+                   "scene_builder": args.builder, "scene_build_ms": round(build_ms, 1),
                    "tuning": dict(tuning, trace_kernel=abi.TRACE_NAMES[tuning["trace_kernel"]]),
                    "autotune_ms": round(tune_ms, 1)}),
         "roofline": roofline,

@@ -47,6 +47,8 @@ def load_library(path=LIB_PATH):
     lib.clv_last_error.argtypes = [vp]
     lib.clv_scene_upload.restype = C.c_int
     lib.clv_scene_upload.argtypes = [vp, vp, C.POINTER(C.c_uint32)]
+    lib.clv_scene_upload_ex.restype = C.c_int
+    lib.clv_scene_upload_ex.argtypes = [vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
     lib.clv_scene_free.restype = C.c_int
     lib.clv_scene_free.argtypes = [vp, C.c_uint32]
     lib.clv_render.restype = C.c_int
@@ -231,11 +233,18 @@ class Renderer:
     def __exit__(self, *a):
         self.close()
 
-    def upload(self, scene):
-        """scene: clovers_amd.scenefile.SceneDescription -> scene id."""
+    def upload(self, scene, builder="host"):
+        """scene: clovers_amd.scenefile.SceneDescription -> scene id.  builder:
+        "host" (binned SAH on the host, clv_scene_upload) or "device" (the GPU
+        builder, clv_scene_upload_ex with CLV_BUILD_DEVICE)."""
         desc, keep = scene.as_ctypes()
         sid = C.c_uint32()
-        self._check(self.lib.clv_scene_upload(self.h, C.byref(desc), C.byref(sid)))
+        if builder == "host":
+            self._check(self.lib.clv_scene_upload(self.h, C.byref(desc), C.byref(sid)))
+        elif builder == "device":
+            self._check(self.lib.clv_scene_upload_ex(self.h, C.byref(desc), abi.BUILD_DEVICE, C.byref(sid)))
+        else:
+            raise ValueError("builder is 'host' or 'device'")
         return sid.value
 
     def free(self, scene_id):

@@ -27,6 +27,7 @@ MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DISPERSIVE, MAT_DIFFUSE_LIGHT, MA
 TEX_SOLID, TEX_ILLUMINANT_D65, TEX_ILLUMINANT_D50, TEX_SPATIAL_CHECKER, TEX_SURFACE_CHECKER = range(5)
 # BVH algorithms (bvh.rs:32-40)
 BVH_LAM, BVH_SAH = 0, 1
+BUILD_HOST, BUILD_DEVICE = 0, 1  # clv_scene_builder
 GUIDE_OFF, GUIDE_ON = 0, 1
 # render modes (render.rs:52-62)
 MODE_PATH_TRACING, MODE_NORMAL_MAP, MODE_BVH_TEST_COUNT, MODE_PRIMITIVE_TEST_COUNT = range(4)

@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <memory>
 #include <new>
@@ -29,6 +30,7 @@
 #include <vector>
 
 #include "../../include/clovers_hip.h"
+#include "bvh_build.h"
 #include "host_build.h"
 #include "kernels.h"
 
@@ -306,7 +308,7 @@ clv_status ensure_tables(clv_ctx *c, uint32_t jobs, uint32_t items) {
 }
 
 /* append `bytes` to an arena array, growing it (a device-to-device copy of
- * what it holds) when needed */
+ * what it holds) when needed; src NULL only reserves the range */
 clv_status arena_append(clv_ctx *c, DevArray &a, const void *src, size_t bytes) {
   const size_t need = a.used + bytes;
   if (need > a.cap) {
@@ -325,7 +327,7 @@ clv_status arena_append(clv_ctx *c, DevArray &a, const void *src, size_t bytes) 
     a.p = p;
     a.cap = cap;
   }
-  if (bytes) HIPCHK(c, hipMemcpyAsync(a.p + a.used, src, bytes, hipMemcpyHostToDevice, c->stream));
+  if (bytes && src) HIPCHK(c, hipMemcpyAsync(a.p + a.used, src, bytes, hipMemcpyHostToDevice, c->stream));
   a.used = need;
   return CLV_OK;
 }
@@ -885,12 +887,55 @@ void clv_destroy(clv_ctx *c) {
 
 const char *clv_last_error(const clv_ctx *c) { return c ? c->err.c_str() : "no context"; }
 
-clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *scene_id) {
-  if (!c || !scene_id) return CLV_EINVAL;
+}  // extern "C"
+
+namespace {
+
+/* the device builder's inputs (records in primitive order, their offsets and
+ * boxes) in temporary device buffers; the tree goes straight into the arena */
+clv_status device_bvh(clv_ctx *c, const HostScene &hs, const ArenaBase &b, DeviceBvhResult *res) {
+  const size_t n = hs.prim_unit.size();
+  const size_t rec_bytes = hs.prims.size() * sizeof(float), unit_bytes = n * 4, box_bytes = hs.prim_box.size() * 4;
+  char *tmp = nullptr;
+  HIPCHK(c, hipMalloc((void **)&tmp, rec_bytes + unit_bytes + box_bytes + 512));
+  float4 *src = reinterpret_cast<float4 *>(tmp);
+  uint32_t *unit = reinterpret_cast<uint32_t *>(tmp + ((rec_bytes + 255) & ~(size_t)255));
+  float4 *box = reinterpret_cast<float4 *>(reinterpret_cast<char *>(unit) + ((unit_bytes + 255) & ~(size_t)255));
+  hipError_t e = hipMemcpyAsync(src, hs.prims.data(), rec_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(unit, hs.prim_unit.data(), unit_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(box, hs.prim_box.data(), box_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    return fail(c, CLV_EDEVICE, std::string("device BVH inputs: ") + hipGetErrorString(e));
+  }
+  DeviceBvhInput in;
+  in.box = box;
+  in.unit = unit;
+  in.src = src;
+  in.n = (uint32_t)n;
+  for (int a = 0; a < 3; a++) {
+    in.clo[a] = hs.clo[a];
+    in.chi[a] = hs.chi[a];
+  }
+  std::string err;
+  /* the traversal keeps at most 3 pending children per level in a 72-entry stack */
+  const int st = device_build_bvh(c->stream, in, reinterpret_cast<Bvh4Node *>(c->arena[AR_NODES].p), b.nodes,
+                                  reinterpret_cast<float4 *>(c->arena[AR_PRIMS].p), b.units, 23u, res, &err);
+  (void)hipFree(tmp);
+  if (st != CLV_OK) return fail(c, (clv_status)st, err);
+  if ((size_t)res->units * 4 != hs.prims.size()) return fail(c, CLV_EINTERNAL, "device BVH build lost primitive records");
+  return CLV_OK;
+}
+
+clv_status upload_scene(clv_ctx *c, const clv_scene_desc *desc, uint32_t builder, uint32_t *scene_id) {
+  if (!c || !scene_id || builder > CLV_BUILD_DEVICE) return CLV_EINVAL;
   HIPCHK(c, hipSetDevice(c->device));
   HostScene hs;
   std::string err;
-  int st = build_host_scene(desc, &hs, &err);
+  const bool on_device = builder == CLV_BUILD_DEVICE;
+  auto t0 = std::chrono::steady_clock::now();
+  int st = build_host_scene(desc, &hs, &err, on_device);
   if (st != CLV_OK) return fail(c, (clv_status)st, err);
   ArenaBase b;
   b.nodes = (uint32_t)(c->arena[AR_NODES].used / sizeof(Bvh4Node));
@@ -913,8 +958,10 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
     const void *p;
     size_t bytes;
   } parts[AR_COUNT] = {
-      {AR_NODES, hs.nodes.data(), hs.nodes.size() * sizeof(Bvh4Node)},
-      {AR_PRIMS, hs.prims.data(), hs.prims.size() * sizeof(float)},
+      /* device build: reserve room for the tree (at most n - 1 nodes) and the records */
+      {AR_NODES, on_device ? nullptr : hs.nodes.data(),
+       (on_device ? std::max<size_t>(hs.prim_unit.size(), 2) - 1 : hs.nodes.size()) * sizeof(Bvh4Node)},
+      {AR_PRIMS, on_device ? nullptr : hs.prims.data(), hs.prims.size() * sizeof(float)},
       {AR_INFO, hs.prim_info.data(), hs.prim_info.size() * sizeof(PrimInfo)},
       {AR_XFORMS, hs.xforms.data(), hs.xforms.size() * sizeof(Xform)},
       {AR_MATS, hs.materials.data(), hs.materials.size() * sizeof(DevMaterial)},
@@ -926,6 +973,11 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
   clv_status r = CLV_OK;
   for (const Part &pt : parts)
     if (r == CLV_OK) r = arena_append(c, c->arena[pt.arr], pt.p, pt.bytes);
+  DeviceBvhResult dres;
+  if (r == CLV_OK && on_device) {
+    r = device_bvh(c, hs, b, &dres);
+    if (r == CLV_OK) c->arena[AR_NODES].used = ((size_t)b.nodes + dres.n_nodes) * sizeof(Bvh4Node);
+  }
   if (r == CLV_OK) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) r = fail(c, CLV_EDEVICE, hipGetErrorString(e));
@@ -942,13 +994,21 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
   s->has_xforms = !hs.xforms.empty();
   for (const DevMaterial &m : hs.materials) s->has_ext = s->has_ext || m.tex != TX_PLAIN || m.cone != 0u;
   s->camera = hs.camera;
-  s->n_nodes = (uint32_t)hs.nodes.size();
+  s->n_nodes = on_device ? dres.n_nodes : (uint32_t)hs.nodes.size();
   s->prim_units = (uint32_t)(hs.prims.size() / 4);
   s->n_prims = hs.n_prims;
   s->prim_bytes = hs.n_prims ? (uint32_t)(hs.prims.size() * 4 / hs.n_prims) : 0;
-  s->depth = hs.bvh_depth;
-  s->build_ms = hs.build_ms;
-  {
+  s->depth = on_device ? dres.depth : hs.bvh_depth;
+  s->build_ms = on_device ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count()
+                          : hs.build_ms;
+  if (on_device) {
+    for (int a = 0; a < 3; a++) {
+      float lo = hs.lo[a], hi = hs.hi[a];
+      if (!(hi - lo > 1e-6f)) { lo -= 1.0f; hi += 1.0f; }
+      s->lo[a] = lo;
+      s->hi[a] = hi;
+    }
+  } else {
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     const Bvh4Node &root = hs.nodes[0];
     for (int k = 0; k < 4; k++) {
@@ -963,11 +1023,22 @@ clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *sc
       s->hi[a] = hi[a];
     }
   }
-  c->counters.build_ms = hs.build_ms;
+  c->counters.build_ms = s->build_ms;
   uint32_t id = c->next_scene++;
   c->scenes[id] = s;
   *scene_id = id;
   return CLV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+clv_status clv_scene_upload(clv_ctx *c, const clv_scene_desc *desc, uint32_t *scene_id) {
+  return upload_scene(c, desc, CLV_BUILD_HOST, scene_id);
+}
+
+clv_status clv_scene_upload_ex(clv_ctx *c, const clv_scene_desc *desc, uint32_t builder, uint32_t *scene_id) {
+  return upload_scene(c, desc, builder, scene_id);
 }
 
 clv_status clv_scene_validate(const clv_scene_desc *desc, clv_scene_info *info) {

@@ -642,7 +642,7 @@ float cone_threshold(float spread_rad) {
   return f32_from_order(hi);
 }
 
-int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
+int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, bool device_bvh) {
   auto t0 = std::chrono::steady_clock::now();
   if (!d || d->abi_version != CLV_ABI_VERSION) { *err = "descriptor missing or ABI version mismatch"; return CLV_EINVAL; }
   if (d->n_objects == 0 || !d->objects) { *err = "scene has no objects"; return CLV_EINVAL; }
@@ -936,20 +936,48 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err) {
   }
   hs->n_prims = (uint32_t)prims.size();
 
-  /* scene BVH */
-  BvhBuilder bb(prims);
-  int root = bb.build(0, (uint32_t)prims.size(), 0);
-  Collapser col(bb, prims);
-  col.out.reserve(bb.nodes.size() / 2 + 1);
-  col.records.reserve(prims.size() * 16);
-  col.collapse(root, 0);
-  renumber_bfs(col.out);
-  hs->nodes.swap(col.out);
-  hs->prims.swap(col.records);
-  hs->bvh_depth = col.depth;
-  /* traversal keeps at most 3 pending children per level in a 72-entry stack */
-  if (3u * (col.depth + 1u) > 72u) { *err = "scene BVH deeper than the traversal stack allows"; return CLV_EUNSUPPORTED; }
-  if (hs->prims.size() / 4 > LEAF_OFFSET_MASK) { *err = "scene too large for 28-bit record offsets"; return CLV_EUNSUPPORTED; }
+  /* scene BVH: on the device (records in primitive order, boxes) or here */
+  if (device_bvh) {
+    hs->device_bvh = true;
+    hs->prim_unit.resize(prims.size());
+    hs->prim_box.resize(8 * prims.size());
+    hs->prims.reserve(prims.size() * 16);
+    for (int a = 0; a < 3; a++) {
+      hs->clo[a] = hs->lo[a] = FLT_MAX;
+      hs->chi[a] = hs->hi[a] = -FLT_MAX;
+    }
+    for (size_t i = 0; i < prims.size(); i++) {
+      hs->prim_unit[i] = (uint32_t)(hs->prims.size() / 4);
+      write_record(hs->prims, prims[i]);
+      const Box &b = prims[i].cull;
+      float *o = &hs->prim_box[8 * i];
+      for (int a = 0; a < 3; a++) {
+        o[a] = b.lo[a];
+        o[4 + a] = b.hi[a];
+        const float c = 0.5f * (b.lo[a] + b.hi[a]);
+        hs->clo[a] = fminf(hs->clo[a], c);
+        hs->chi[a] = fmaxf(hs->chi[a], c);
+        hs->lo[a] = fminf(hs->lo[a], b.lo[a]);
+        hs->hi[a] = fmaxf(hs->hi[a], b.hi[a]);
+      }
+      o[3] = o[7] = 0.0f;
+    }
+    if (hs->prims.size() / 4 > LEAF_OFFSET_MASK) { *err = "scene too large for 28-bit record offsets"; return CLV_EUNSUPPORTED; }
+  } else {
+    BvhBuilder bb(prims);
+    int root = bb.build(0, (uint32_t)prims.size(), 0);
+    Collapser col(bb, prims);
+    col.out.reserve(bb.nodes.size() / 2 + 1);
+    col.records.reserve(prims.size() * 16);
+    col.collapse(root, 0);
+    renumber_bfs(col.out);
+    hs->nodes.swap(col.out);
+    hs->prims.swap(col.records);
+    hs->bvh_depth = col.depth;
+    /* traversal keeps at most 3 pending children per level in a 72-entry stack */
+    if (3u * (col.depth + 1u) > 72u) { *err = "scene BVH deeper than the traversal stack allows"; return CLV_EUNSUPPORTED; }
+    if (hs->prims.size() / 4 > LEAF_OFFSET_MASK) { *err = "scene too large for 28-bit record offsets"; return CLV_EUNSUPPORTED; }
+  }
 
   /* light tree */
   LightTreeBuilder lt;

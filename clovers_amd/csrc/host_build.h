@@ -28,11 +28,19 @@ struct HostScene {
   uint32_t bvh_depth = 0;
   uint32_t n_objects = 0;
   double build_ms = 0.0;
+  /* built for the device builder: no nodes; prims holds the records in
+   * primitive order, with each one's offset (float4 units) and culling box */
+  bool device_bvh = false;
+  std::vector<uint32_t> prim_unit;
+  std::vector<float> prim_box;        /* 8 floats per primitive: lo.xyz, 0, hi.xyz, 0 */
+  float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};  /* bounds of the box centres */
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};    /* bounds of the boxes */
 };
 
 /* Build everything the device needs from the descriptor.  Returns a
- * clv_status; *err holds the detail on failure. */
-int build_host_scene(const clv_scene_desc *desc, HostScene *out, std::string *err);
+ * clv_status; *err holds the detail on failure.  device_bvh: leave the scene
+ * BVH to the device builder (records in primitive order, boxes, no nodes). */
+int build_host_scene(const clv_scene_desc *desc, HostScene *out, std::string *err, bool device_bvh = false);
 
 /* the f32 threshold below which a ConeLight of this spread (radians) is dark */
 float cone_threshold(float spread_rad);

@@ -221,6 +221,17 @@ const char *clv_last_error(const clv_ctx *ctx);
  * Replaces SceneFile::initialize's BVH builds (scenefile.rs:104-109).
  * The scene stays resident in HBM until clv_scene_free or clv_destroy. */
 clv_status clv_scene_upload(clv_ctx *ctx, const clv_scene_desc *desc, uint32_t *scene_id);
+
+/* clv_scene_upload with a choice of scene-BVH builder.  CLV_BUILD_HOST is the
+ * host binned-SAH build of clv_scene_upload.  CLV_BUILD_DEVICE packs the
+ * primitive records on the host and builds the tree on the GPU (Morton sort,
+ * PLOC clustering, breadth-first collapse into 4-wide nodes): the same node
+ * and record layout, a different tree.  Renders agree with the host build's
+ * up to ties between equally distant hits.  A device-built tree deeper than
+ * the traversal stack allows is CLV_EUNSUPPORTED.  No reference counterpart:
+ * the reference builds on the host only (bvh/build/surface_area_heuristic.rs:18-156). */
+typedef enum { CLV_BUILD_HOST = 0, CLV_BUILD_DEVICE = 1 } clv_scene_builder;
+clv_status clv_scene_upload_ex(clv_ctx *ctx, const clv_scene_desc *desc, uint32_t builder, uint32_t *scene_id);
 clv_status clv_scene_free(clv_ctx *ctx, uint32_t scene_id);
 
 /* Host-only scene check, no device needed: runs the full host build (spectra,

@@ -1,0 +1,130 @@
+"""The GPU scene-BVH builder (clv_scene_upload_ex with CLV_BUILD_DEVICE)
+against the host builder on the same scenes.
+
+The two trees differ, the hits must not: a closest-hit traversal returns the
+same primitive whatever the tree, except where two primitives are hit at the
+same distance (shared mesh edges), where the first one found wins.  Renders of
+scenes without such ties are bit-identical; on meshes the primary hits agree
+at all but a few pixels and the distances agree wherever the ids do."""
+import numpy as np
+import pytest
+
+from clovers_amd import abi, synth
+from clovers_amd.scenefile import load_scene
+from special_scenes import CAM
+
+pytestmark = pytest.mark.gpu
+
+
+def both(r, scene, fn):
+    out = []
+    for builder in ("host", "device"):
+        sid = r.upload(scene, builder=builder)
+        try:
+            out.append(fn(sid))
+        finally:
+            r.free(sid)
+    return out
+
+
+def lamp_scene(objects, materials=None):
+    mats = [{"name": "lamp", "kind": "DiffuseLight", "emit": {"kind": "SolidColor", "color": {"xyz_e": [4, 4, 4]}}},
+            {"name": "grey", "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": {"xyz_e": [0.5, 0.5, 0.5]}}}]
+    return load_scene({"time_0": 0, "time_1": 1, "camera": CAM, "background_color": [0.2, 0.2, 0.2],
+                       "objects": objects, "materials": mats + (materials or [])})
+
+
+def sphere(c, r, mat="grey", priority=False):
+    return {"kind": "Sphere", "center": list(c), "radius": r, "material": mat, "priority": priority}
+
+
+def test_sphere_field_renders_identically(renderer):
+    """488 spheres (Lambertian, metal, glass), 3 bounces: every ray of every
+    bounce finds the same sphere, so the images are equal bit for bit."""
+    o = abi.render_opts(160, 90, 4, 8, seed=3)
+    a, b = both(renderer, synth.spheres(), lambda sid: renderer.render(sid, o)[0])
+    assert np.array_equal(a, b)
+
+
+def test_cornell_renders_identically(renderer):
+    """Instanced boxes (Translate / RotateY chains) and quads."""
+    o = abi.render_opts(96, 96, 4, 8, seed=2)
+    a, b = both(renderer, synth.cornell(), lambda sid: renderer.render(sid, o)[0])
+    assert np.array_equal(a, b)
+
+
+def test_mesh_primary_hits(renderer):
+    """81,920-triangle noise-displaced icosphere: primary hit ids agree at
+    >= 99.99% of pixels, distances wherever the ids agree."""
+    o = abi.render_opts(320, 320, 1, 1, seed=1)
+    (ia, ta), (ib, tb) = both(renderer, synth.mesh(freq=64), lambda sid: renderer.primary_hits(sid, o))
+    same = ia == ib
+    assert same.mean() >= 0.9999, same.mean()
+    assert np.array_equal(ta[same], tb[same])
+    assert (ia != 0xFFFFFFFF).mean() > 0.2  # the mesh covers a good part of the view
+
+
+def test_mesh_render_statistics(renderer):
+    """Full transport on the mesh: the two images agree within their sampling
+    noise (ties on shared edges can send a few paths different ways)."""
+    o = abi.render_opts(96, 96, 32, 6, seed=4)
+    (a, sa), (b, sb) = both(renderer, synth.mesh(freq=32),
+                            lambda sid: renderer.render(sid, o, want_sq=True))
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    diff = a - b
+    assert np.count_nonzero(diff) <= 0.01 * diff.size
+    var = (np.maximum(sa - a * a, 0) + np.maximum(sb - b * b, 0)) / o.samples
+    z = diff.sum() / np.sqrt(var.sum() + 1e-30)
+    assert abs(z) < 4.0
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9, 257])
+def test_small_and_degenerate_scenes(renderer, n):
+    """Scenes of 1 to 257 primitives, all at one point except the lamp (equal
+    Morton codes, zero-area unions and ties in the clustering): hits and
+    images equal the host build's."""
+    objs = [sphere((0, 0, 0), 0.5, "lamp", priority=True)]
+    objs += [sphere((0.25, 0.1, 0.2), 0.3) for _ in range(n - 1)]
+    o = abi.render_opts(48, 48, 4, 4, seed=7)
+    s = lamp_scene(objs)
+    (ha, hb) = both(renderer, s, lambda sid: renderer.primary_hits(sid, o)[1])
+    assert np.array_equal(ha, hb)
+    a, b = both(renderer, s, lambda sid: renderer.render(sid, o)[0])
+    assert np.array_equal(a, b)
+
+
+def test_tree_quality(renderer):
+    """The clustering tree is a SAH-quality tree: node visits per primary ray
+    (counting traversal) within 1.5x of the host binned-SAH build's."""
+    o = abi.render_opts(128, 128, 1, 1, seed=1)
+    a, b = both(renderer, synth.mesh(freq=64),
+                lambda sid: renderer.render_debug(sid, o, abi.MODE_BVH_TEST_COUNT))
+    va, vb = float(a.sum()), float(b.sum())
+    assert va > 0 and vb > 0
+    assert vb <= 1.5 * va, (va, vb)
+
+
+def test_device_build_repeated_uploads(renderer):
+    """Scenes built on the device sit in the arena beside host-built ones:
+    interleaved uploads keep every scene's hits, and freeing the last one
+    returns its range."""
+    o = abi.render_opts(64, 64, 1, 1, seed=1)
+    s1, s2 = synth.spheres(), synth.mesh(freq=16)
+    h1 = renderer.upload(s1, builder="host")
+    d2 = renderer.upload(s2, builder="device")
+    d1 = renderer.upload(s1, builder="device")
+    try:
+        ref1 = renderer.primary_hits(h1, o)
+        got1 = renderer.primary_hits(d1, o)
+        assert np.array_equal(ref1[1], got1[1])
+        h2 = renderer.upload(s2, builder="host")
+        try:
+            i_h, t_h = renderer.primary_hits(h2, o)
+            i_d, t_d = renderer.primary_hits(d2, o)
+            assert (i_h == i_d).mean() >= 0.999
+        finally:
+            renderer.free(h2)
+    finally:
+        for sid in (d1, d2, h1):
+            renderer.free(sid)
