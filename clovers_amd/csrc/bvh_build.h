@@ -6,9 +6,10 @@
  * host builder does the same with binned SAH.  The device builder produces the
  * same 4-wide node and leaf-record layout in HBM directly:
  *
- *   1. 30-bit Morton codes of the primitives' box centres, sorted with a
- *      stable LSD radix sort (8-bit digits; per-tile histograms, one scan,
- *      a stable scatter ranked with wave ballots);
+ *   1. 63-bit Morton codes of the primitives' box centres (21 bits per axis
+ *      over the centres' bounding cube), sorted with a stable LSD radix sort
+ *      (8-bit digits; per-tile histograms, one scan, a stable scatter ranked
+ *      with wave ballots);
  *   2. agglomerative clustering over the sorted order (PLOC: Meister and
  *      Bittner 2018, "Parallel Locally-Ordered Clustering for Bounding Volume
  *      Hierarchy Construction"): every cluster finds, among the 16 on either

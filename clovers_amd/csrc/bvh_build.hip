@@ -36,27 +36,33 @@ __device__ __forceinline__ float4 bb_max(float4 a, float4 b) {
 }
 
 /* ------------------------------------------------------------ Morton --- */
-__device__ __forceinline__ uint32_t expand10(uint32_t x) {
-  x = (x | (x << 16)) & 0x030000FFu;
-  x = (x | (x << 8)) & 0x0300F00Fu;
-  x = (x | (x << 4)) & 0x030C30C3u;
-  x = (x | (x << 2)) & 0x09249249u;
+/* 21 bits of x spread to every third bit of a 63-bit word */
+__device__ __forceinline__ uint64_t expand21(uint32_t v) {
+  uint64_t x = v & 0x1FFFFFu;
+  x = (x | (x << 32)) & 0x001F00000000FFFFull;
+  x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+  x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+  x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
   return x;
 }
-__device__ __forceinline__ uint32_t quant10(float c, float lo, float s) {
-  const float q = fminf(fmaxf((c - lo) * s, 0.0f), 1023.0f);
+__device__ __forceinline__ uint32_t quant21(float c, float lo, float s) {
+  const float q = fminf(fmaxf((c - lo) * s, 0.0f), 2097151.0f);
   return (uint32_t)q;
 }
 
+/* 63-bit Morton code of the box centre, as low and high words */
 __global__ __launch_bounds__(256) void k_bb_morton(uint32_t n, const float4 *__restrict__ box, float lx, float ly,
-                                                   float lz, float sx, float sy, float sz, uint32_t *__restrict__ keys,
-                                                   uint32_t *__restrict__ vals) {
+                                                   float lz, float sx, float sy, float sz, uint32_t *__restrict__ klo,
+                                                   uint32_t *__restrict__ khi, uint32_t *__restrict__ vals) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= n) return;
   const float4 lo = box[2u * i], hi = box[2u * i + 1u];
-  const uint32_t qx = quant10(0.5f * (lo.x + hi.x), lx, sx), qy = quant10(0.5f * (lo.y + hi.y), ly, sy),
-                 qz = quant10(0.5f * (lo.z + hi.z), lz, sz);
-  keys[i] = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+  const uint32_t qx = quant21(0.5f * (lo.x + hi.x), lx, sx), qy = quant21(0.5f * (lo.y + hi.y), ly, sy),
+                 qz = quant21(0.5f * (lo.z + hi.z), lz, sz);
+  const uint64_t key = (expand21(qx) << 2) | (expand21(qy) << 1) | expand21(qz);
+  klo[i] = (uint32_t)key;
+  khi[i] = (uint32_t)(key >> 32);
   vals[i] = i;
 }
 
@@ -139,13 +145,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(uint32_t n, const uint32
   hist[tid * nb + blockIdx.x] = cnt[tid]; /* digit-major: one scan gives every (digit, tile) offset */
 }
 
-/* stable scatter: elements of a tile are ranked in index order, a chunk of
- * 256 at a time; within a wave, lanes with the same digit find each other
- * with 8 ballots */
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(uint32_t n, const uint32_t *__restrict__ kin,
-                                                           const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
-                                                           uint32_t *__restrict__ vout, uint32_t shift, uint32_t nb,
-                                                           const uint32_t *__restrict__ hist) {
+/* stable scatter of (key low word, key high word, value): elements of a tile
+ * are ranked in index order, a chunk of 256 at a time; within a wave, lanes
+ * with the same digit find each other with 8 ballots */
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(uint32_t n, const uint32_t *__restrict__ lin,
+                                                           const uint32_t *__restrict__ hin, const uint32_t *__restrict__ vin,
+                                                           uint32_t *__restrict__ lout, uint32_t *__restrict__ hout,
+                                                           uint32_t *__restrict__ vout, uint32_t high, uint32_t shift,
+                                                           uint32_t nb, const uint32_t *__restrict__ hist) {
   __shared__ uint32_t base[256];
   __shared__ uint32_t wc[RS_THREADS / 64][256];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -156,8 +163,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(uint32_t n, const uin
     __syncthreads();
     const uint32_t i = blockIdx.x * RS_TILE + k * RS_THREADS + tid;
     const bool valid = i < n;
-    const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
-    const uint32_t d = (key >> shift) & 255u;
+    const uint32_t kl = valid ? lin[i] : 0u, kh = valid ? hin[i] : 0u, val = valid ? vin[i] : 0u;
+    const uint32_t d = ((high ? kh : kl) >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -171,7 +178,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(uint32_t n, const uin
     if (valid) {
       uint32_t pos = base[d] + rank;
       for (uint32_t ww = 0; ww < w; ww++) pos += wc[ww][d];
-      kout[pos] = key;
+      lout[pos] = kl;
+      hout[pos] = kh;
       vout[pos] = val;
     }
     __syncthreads();
@@ -477,15 +485,17 @@ int device_build_bvh(hipStream_t st, const DeviceBvhInput &in, Bvh4Node *nodes, 
   /* sizes of the pieces, counted once to allocate and again to carve */
   auto layout = [&](Carve &c) {
     struct P {
-      uint32_t *k0, *v0, *k1, *v1, *hist, *nn, *cid0, *cid1, *F0, *F1;
+      uint32_t *k0, *h0, *v0, *k1, *h1, *v1, *hist, *nn, *cid0, *cid1, *F0, *F1;
       uint64_t *bsum, *tot, *fl;
       float4 *cb0, *cb1, *bbox;
       uint2 *bch;
       uint4 *bmeta, *plan;
     } p;
     p.k0 = c.take<uint32_t>(n);
+    p.h0 = c.take<uint32_t>(n);
     p.v0 = c.take<uint32_t>(n);
     p.k1 = c.take<uint32_t>(n);
+    p.h1 = c.take<uint32_t>(n);
     p.v1 = c.take<uint32_t>(n);
     p.hist = c.take<uint32_t>(256ull * nbR);
     p.nn = c.take<uint32_t>(n);
@@ -526,26 +536,30 @@ int device_build_bvh(hipStream_t st, const DeviceBvhInput &in, Bvh4Node *nodes, 
     }                                                                         \
   } while (0)
   {
-    /* 1. Morton codes, stable radix sort (4 passes, result back in k0/v0) */
+    /* 1. 63-bit Morton codes (one scale for all axes: the centres' largest
+     * extent), stable radix sort (8 passes, result back in k0/h0/v0) */
     float s[3];
-    for (int a = 0; a < 3; a++) {
-      const float ext = in.chi[a] - in.clo[a];
-      s[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
-    }
+    float ext = 0.0f;
+    for (int a = 0; a < 3; a++) ext = fmaxf(ext, in.chi[a] - in.clo[a]);
+    for (int a = 0; a < 3; a++) s[a] = ext > 0.0f ? 2097152.0f / ext : 0.0f;
     const uint32_t g = cdiv(n, 256);
     hipLaunchKernelGGL(k_bb_morton, dim3(g), dim3(256), 0, st, n, in.box, in.clo[0], in.clo[1], in.clo[2], s[0], s[1],
-                       s[2], P.k0, P.v0);
+                       s[2], P.k0, P.h0, P.v0);
     BBCHK(hipGetLastError());
-    uint32_t *kin = P.k0, *vin = P.v0, *kout = P.k1, *vout = P.v1;
-    for (uint32_t shift = 0; shift < 32; shift += 8) {
-      hipLaunchKernelGGL(k_rs_hist, dim3(nbR), dim3(RS_THREADS), 0, st, n, (const uint32_t *)kin, shift, nbR, P.hist);
+    uint32_t *lin = P.k0, *hin = P.h0, *vin = P.v0, *lout = P.k1, *hout = P.h1, *vout = P.v1;
+    for (uint32_t pass = 0; pass < 8; pass++) {
+      const uint32_t high = pass >> 2, shift = 8u * (pass & 3u);
+      hipLaunchKernelGGL(k_rs_hist, dim3(nbR), dim3(RS_THREADS), 0, st, n, (const uint32_t *)(high ? hin : lin), shift,
+                         nbR, P.hist);
       BBCHK(hipGetLastError());
       BBCHK(scan<uint32_t>(st, P.hist, 256u * nbR, reinterpret_cast<uint32_t *>(P.bsum),
                            reinterpret_cast<uint32_t *>(P.tot)));
-      hipLaunchKernelGGL(k_rs_scatter, dim3(nbR), dim3(RS_THREADS), 0, st, n, (const uint32_t *)kin,
-                         (const uint32_t *)vin, kout, vout, shift, nbR, (const uint32_t *)P.hist);
+      hipLaunchKernelGGL(k_rs_scatter, dim3(nbR), dim3(RS_THREADS), 0, st, n, (const uint32_t *)lin,
+                         (const uint32_t *)hin, (const uint32_t *)vin, lout, hout, vout, high, shift, nbR,
+                         (const uint32_t *)P.hist);
       BBCHK(hipGetLastError());
-      std::swap(kin, kout);
+      std::swap(lin, lout);
+      std::swap(hin, hout);
       std::swap(vin, vout);
     }
     /* 2. clustering */
