@@ -18,11 +18,12 @@ STATUS_NAMES = {0: "CLV_OK", 1: "CLV_EINVAL", 2: "CLV_ENOMEM", 3: "CLV_EDEVICE",
 
 # object kinds (objects.rs:59-87)
 OBJ_SPHERE, OBJ_QUAD, OBJ_TRIANGLE, OBJ_BOXY, OBJ_TRIANGLE_QUV, OBJ_MOVING_SPHERE = range(6)
+OBJ_MEDIUM_SPHERE, OBJ_MEDIUM_BOXY = 6, 7  # constant media: boundary params + density
 # xform ops (objects/translate.rs, objects/rotate.rs)
 XF_TRANSLATE, XF_ROTATE_Y = 0, 1
 MAX_XFORM_OPS = 4
 # material kinds (materials.rs:141-156)
-MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DISPERSIVE, MAT_DIFFUSE_LIGHT, MAT_CONE_LIGHT = range(6)
+MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DISPERSIVE, MAT_DIFFUSE_LIGHT, MAT_CONE_LIGHT, MAT_ISOTROPIC = range(7)
 # texture kinds (textures.rs:22-109)
 TEX_SOLID, TEX_ILLUMINANT_D65, TEX_ILLUMINANT_D50, TEX_SPATIAL_CHECKER, TEX_SURFACE_CHECKER = range(5)
 # BVH algorithms (bvh.rs:32-40)

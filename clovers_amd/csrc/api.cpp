@@ -57,7 +57,8 @@ struct SceneRes {
   uint32_t n_lights = 0;
   int32_t bg_spd = -1;
   bool has_xforms = false;
-  bool has_ext = false;      /* checker textures or cone lights */
+  bool has_ext = false;      /* checker textures, cone lights or constant media */
+  uint32_t media_off = 0, n_media = 0;  /* constant-medium records (arena units) */
   clv_camera_init camera;
   uint32_t n_nodes = 0, n_prims = 0, prim_bytes = 0, depth = 0;
   uint32_t prim_units = 0;   /* float4 units of primitive records */
@@ -220,8 +221,12 @@ struct TraceGeom {
   uint32_t lds_node_base = 0, lds_nodes = 0, lds_unit_base = 0, lds_units = 0;
 };
 
-clv_status trace_geometry(clv_ctx *c, const SceneRes *s, uint32_t full_grid_paths, TraceGeom *g) {
-  if (c->trace_variant == TRACE_LDS) {
+/* the traversal kernel a batch runs: the tuned one, or the plain one when a
+ * scene of the batch has constant media (only it has the media pass) */
+int trace_variant_for(const clv_ctx *c, bool media) { return media ? TRACE_SIMPLE : c->trace_variant; }
+
+clv_status trace_geometry(clv_ctx *c, const SceneRes *s, uint32_t full_grid_paths, TraceGeom *g, int variant) {
+  if (variant == TRACE_LDS) {
     size_t budget = c->lds_block_bytes > lds_trace_stack_bytes() ? c->lds_block_bytes - lds_trace_stack_bytes() : 0;
     g->lds_node_base = s->base.nodes;
     g->lds_unit_base = s->base.units;
@@ -233,7 +238,7 @@ clv_status trace_geometry(clv_ctx *c, const SceneRes *s, uint32_t full_grid_path
     return ensure_spill(c, lds_trace_spill_words(g->blocks));
   }
   g->blocks = c->trace_blocks;
-  if (full_grid_paths && c->full_grid && c->trace_variant != TRACE_PERSISTENT) g->blocks = (full_grid_paths + 255u) / 256u;
+  if (full_grid_paths && c->full_grid && variant != TRACE_PERSISTENT) g->blocks = (full_grid_paths + 255u) / 256u;
   return ensure_spill(c, trace_spill_words(g->blocks));
 }
 
@@ -404,6 +409,8 @@ DevJob make_job(const SceneRes *s, const clv_camera_init &cam, const clv_render_
     j.lo[a] = s->lo[a];
     j.hi[a] = s->hi[a];
   }
+  j.media_off = s->media_off;
+  j.n_media = s->n_media;
   return j;
 }
 
@@ -512,7 +519,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   std::vector<const SceneRes *> sres(n);
   uint64_t pixels = 0, samples = 0;
   uint32_t flags = 0;
-  bool any_xform = false, any_ext = false;
+  bool any_xform = false, any_ext = false, any_media = false;
   for (uint32_t k = 0; k < n; k++) {
     const clv_job &jb = jobs[k];
     CHK(check_opts(c, &jb.opts));
@@ -529,6 +536,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     flags |= jb.opts.flags;
     any_xform = any_xform || it->second->has_xforms;
     any_ext = any_ext || it->second->has_ext;
+    any_media = any_media || it->second->n_media > 0;
   }
   if (pixels > (1ull << 28)) return fail(c, CLV_EINVAL, "batch larger than 2^28 pixels");
   bool any_guide = false;
@@ -557,13 +565,15 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   for (uint32_t k = 1; k < n; k++)
     if ((uint64_t)dj[k].width * dj[k].height * dj[k].spp > (uint64_t)dj[big].width * dj[big].height * dj[big].spp) big = k;
   TraceGeom geom;
-  CHK(trace_geometry(c, sres[big], pl.max_paths, &geom));
+  const int variant = trace_variant_for(c, any_media);
+  CHK(trace_geometry(c, sres[big], pl.max_paths, &geom, variant));
   HIPCHK(c, hipMemcpyAsync(c->jobs, dj.data(), n * sizeof(DevJob), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->items, pl.items.data(), pl.items.size() * sizeof(DevItem), hipMemcpyHostToDevice,
                            c->stream));
   DevScene ds = arena_view(c);
   ds.has_xforms = any_xform ? 1u : 0u;
   ds.has_ext = any_ext ? 1u : 0u;
+  ds.has_media = any_media ? 1u : 0u;
   const bool timing = (flags & CLV_FLAG_TIME_KERNELS) != 0;
   const bool counting = (flags & CLV_FLAG_COUNT) != 0;
   uint32_t extends = 0;
@@ -646,10 +656,10 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
       ta.spill = c->spill;
       ta.stats = (unsigned long long *)c->stats;
       fill_trace_args(&ta, geom);
-      if (c->trace_variant == TRACE_LDS)
+      if (variant == TRACE_LDS)
         launch_trace_lds(c->stream, ds, bv, ta, geom.blocks, geom.lds_bytes, counting, 0);
       else
-        launch_extend(c->stream, ds, bv, ta, geom.blocks, counting, c->trace_variant);
+        launch_extend(c->stream, ds, bv, ta, geom.blocks, counting, variant);
       if (timing) HIPCHK(c, hipEventRecord(c->events[ev + 1], c->stream));
       ev += timing ? 2 : 0;
       if (any_guide && ga.alpha > 0.0f) {
@@ -689,7 +699,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
     add(pl.res.data(), pl.res.size() * 4);
     add(pl.depth.data(), pl.depth.size() * 4);
     add(pl.train.data(), pl.train.size());
-    const uint32_t sc[10] = {flags, any_sq, any_guide, frozen, (uint32_t)c->trace_variant, c->grid_cap, ds.has_xforms,
+    const uint32_t sc[10] = {flags, any_sq, any_guide, frozen, (uint32_t)variant, c->grid_cap, ds.has_xforms,
                              gs.train_spp, ds.has_ext, gs.credit};
     add(sc, sizeof(sc));
     const float gf[4] = {gs.alpha, gs.lr, gs.wclip, (float)gs.records};
@@ -924,7 +934,13 @@ clv_status device_bvh(clv_ctx *c, const HostScene &hs, const ArenaBase &b, Devic
                                   reinterpret_cast<float4 *>(c->arena[AR_PRIMS].p), b.units, 23u, res, &err);
   (void)hipFree(tmp);
   if (st != CLV_OK) return fail(c, (clv_status)st, err);
-  if ((size_t)res->units * 4 != hs.prims.size()) return fail(c, CLV_EINTERNAL, "device BVH build lost primitive records");
+  const size_t tree_units = hs.prims.size() / 4 - hs.media_units;
+  if (res->units != tree_units) return fail(c, CLV_EINTERNAL, "device BVH build lost primitive records");
+  /* the media records follow the tree's */
+  if (hs.media_units)
+    HIPCHK(c, hipMemcpyAsync(reinterpret_cast<float4 *>(c->arena[AR_PRIMS].p) + b.units + tree_units,
+                             hs.prims.data() + 4 * tree_units, (size_t)hs.media_units * 16u, hipMemcpyHostToDevice,
+                             c->stream));
   return CLV_OK;
 }
 
@@ -992,11 +1008,14 @@ clv_status upload_scene(clv_ctx *c, const clv_scene_desc *desc, uint32_t builder
   s->light_root = hs.light_root;
   s->bg_spd = hs.background_spd;
   s->has_xforms = !hs.xforms.empty();
-  for (const DevMaterial &m : hs.materials) s->has_ext = s->has_ext || m.tex != TX_PLAIN || m.cone != 0u;
+  for (const DevMaterial &m : hs.materials)
+    s->has_ext = s->has_ext || m.tex != TX_PLAIN || m.cone != 0u || m.kind == CLV_MAT_ISOTROPIC;
   s->camera = hs.camera;
   s->n_nodes = on_device ? dres.n_nodes : (uint32_t)hs.nodes.size();
   s->prim_units = (uint32_t)(hs.prims.size() / 4);
   s->n_prims = hs.n_prims;
+  s->n_media = hs.n_media;
+  s->media_off = b.units + (uint32_t)(hs.prims.size() / 4) - hs.media_units;
   s->prim_bytes = hs.n_prims ? (uint32_t)(hs.prims.size() * 4 / hs.n_prims) : 0;
   s->depth = on_device ? dres.depth : hs.bvh_depth;
   s->build_ms = on_device ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count()
@@ -1387,6 +1406,7 @@ clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   DevScene ds = arena_view(c);
   ds.has_xforms = it->second->has_xforms ? 1u : 0u;
   ds.has_ext = it->second->has_ext ? 1u : 0u;
+  ds.has_media = it->second->n_media > 0 ? 1u : 0u;
   GenParams gp;
   memset(&gp, 0, sizeof(gp));
   gp.items = c->items;
@@ -1401,13 +1421,14 @@ clv_status clv_primary_hits(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   ta.ids = c->ids;
   ta.ts = c->ts;
   TraceGeom geom;
-  CHK(trace_geometry(c, it->second, 0, &geom));
+  const int variant = trace_variant_for(c, it->second->n_media > 0);
+  CHK(trace_geometry(c, it->second, 0, &geom, variant));
   ta.spill = c->spill;
   fill_trace_args(&ta, geom);
-  if (c->trace_variant == TRACE_LDS)
+  if (variant == TRACE_LDS)
     launch_trace_lds(c->stream, ds, c->pb, ta, geom.blocks, geom.lds_bytes, false, 1);
   else
-    launch_primary(c->stream, ds, c->pb, ta, geom.blocks, c->trace_variant);
+    launch_primary(c->stream, ds, c->pb, ta, geom.blocks, variant);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(ids, c->ids, sizeof(uint32_t) * WH, hipMemcpyDeviceToHost, c->stream));
   if (ts) HIPCHK(c, hipMemcpyAsync(ts, c->ts, sizeof(float) * WH, hipMemcpyDeviceToHost, c->stream));
@@ -1435,6 +1456,7 @@ clv_status clv_render_debug(clv_ctx *c, uint32_t scene_id, const clv_render_opts
   DevScene ds = arena_view(c);
   ds.has_xforms = it->second->has_xforms ? 1u : 0u;
   ds.has_ext = it->second->has_ext ? 1u : 0u;
+  ds.has_media = it->second->n_media > 0 ? 1u : 0u;
   DebugArgs da;
   memset(&da, 0, sizeof(da));
   da.width = W;

@@ -203,9 +203,9 @@ __device__ __forceinline__ BMeta child_meta(uint32_t id, const uint4 *__restrict
                                             const float4 *__restrict__ src) {
   BMeta r;
   if (id & BB_LEAF) {
-    const uint32_t hdr = __float_as_uint(src[unit[id & ~BB_LEAF]].w);
+    const float4 f0 = src[unit[id & ~BB_LEAF]];
     r.count = 1u;
-    r.units = PREC_OF(hdr & 3u);
+    r.units = PREC_UNITS(__float_as_uint(f0.w), __float_as_uint(f0.z));
     r.cost = 1.0f;
     r.leafy = 1u;
   } else {
@@ -417,7 +417,8 @@ __global__ __launch_bounds__(256) void k_cl_emit(uint32_t m, const uint4 *__rest
         const uint32_t x = stk[--sp];
         if (x & BB_LEAF) {
           const uint32_t q = x & ~BB_LEAF, s0 = unit[q];
-          const uint32_t nu = PREC_OF(__float_as_uint(src[s0].w) & 3u);
+          const float4 f0 = src[s0];
+          const uint32_t nu = PREC_UNITS(__float_as_uint(f0.w), __float_as_uint(f0.z));
           for (uint32_t u = 0; u < nu; u++) dst[unit_base + u_off + u] = src[s0 + u];
           u_off += nu;
           cnt++;

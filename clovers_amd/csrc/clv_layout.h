@@ -9,8 +9,12 @@
  *                sphere 2 x float4 (32 B), moving sphere 3 x float4 (48 B:
  *                centre at time 0, radius, centre_1 - centre_0), triangle
  *                4 x float4 (64 B: corners, plane d, unit normal),
- *                quad 5 x float4 (80 B); word .w of float4 #0 is the header
- *              (type | (xform+1) << 2), word .w of float4 #1 the primitive
+ *                quad 5 x float4 (80 B), constant medium 3 x float4 with a
+ *                sphere boundary or 32 x float4 with a box boundary (its six
+ *                quad records; media are not in the BVH: each scene's media
+ *                records follow its leaf records and every ray tests them
+ *                after the tree); word .w of float4 #0 is the header
+ *              (type | (xform+1) << 3), word .w of float4 #1 the primitive
  *              index into prim_info.
  *   prim_info  per primitive: material, reported id (object*8 + face).
  *   xforms     instance chains (RotateY/Translate), outermost op first.
@@ -32,15 +36,32 @@
 #define PT_QUAD 1u
 #define PT_TRI 2u
 #define PT_MSPHERE 3u
+#define PT_MEDIUM 4u
+#define PT_TYPE_MASK 7u   /* header bits of the type; the xform index + 1 sits above them */
+#define PT_XF_SHIFT 3u
 
 /* record sizes in float4 units */
 #define PREC_SPHERE 2u
 #define PREC_TRI 4u
 #define PREC_QUAD 5u
 #define PREC_MSPHERE 3u
-/* record size by type as a nibble table: PT_SPHERE 2, PT_QUAD 5, PT_TRI 4, PT_MSPHERE 3 */
+/* record size by type as a nibble table: PT_SPHERE 2, PT_QUAD 5, PT_TRI 4, PT_MSPHERE 3
+ * (a medium's size is in its record: see PREC_UNITS) */
 #define PREC_TABLE 0x3452u
 #define PREC_OF(type) ((PREC_TABLE >> ((type) << 2)) & 0xFu)
+
+/* constant medium record (objects/constant_medium.rs):
+ *   [-1/density, boundary kind, record units, header] [0, 0, 0, info]
+ *   then the boundary: MED_SPHERE [centre, radius]; MED_BOX the six Boxy quads
+ *   (boxy.rs:43-83) as quad records, in the reference's face order */
+#define MED_SPHERE 0u
+#define MED_BOX 1u
+#define PREC_MED_SPHERE 3u
+#define PREC_MED_BOX (2u + 6u * PREC_QUAD)
+/* units of the record at rec (float4 units) whose header word is hdr; the
+ * medium's size word is float4 #0 .z */
+#define PREC_UNITS(hdr, size_word) \
+  (((hdr) & PT_TYPE_MASK) == PT_MEDIUM ? (size_word) : PREC_OF((hdr) & PT_TYPE_MASK))
 
 /* BVH4 child word: interior node index, or a leaf, or empty */
 #define CHILD_LEAF 0x80000000u
@@ -102,7 +123,7 @@ typedef struct {
   float cone_cos;        /* ConeLight: c = -d.n / (|d| |n|) in [-1, 1] is outside the cone iff
                             c < cone_cos (the f32 threshold of acos(c) > spread in radians) */
   uint32_t cone;         /* 1: ConeLight */
-} DevMaterial;           /* 64 bytes */
+} DevMaterial;           /* 64 bytes; Isotropic is kind CLV_MAT_ISOTROPIC in class MC_LAMBERTIAN */
 
 /* light (MIS target) record: the reference's Quad/Triangle/Sphere params */
 typedef struct {
@@ -144,6 +165,8 @@ typedef struct {
   uint32_t fp64;          /* accumulate in f64 */
   uint32_t guiding;        /* clv_guiding */
   float lo[3], hi[3];      /* scene box: guiding inputs are positions normalised to it */
+  uint32_t media_off;      /* the scene's constant-medium records: prims[media_off ...], n_media of them */
+  uint32_t n_media;
 } DevJob;
 
 /* one job's share of a wavefront pass: `samples` samples of every pixel */

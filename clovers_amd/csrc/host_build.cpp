@@ -176,6 +176,7 @@ struct FlatPrim {
   uint32_t xform;     /* device xform index + 1, 0 = none */
   uint32_t info;      /* index into prim_info */
   float d[20];        /* record payload (see write_record) */
+  uint32_t ext, ext_n;  /* media: the boundary's floats in the side store */
   Box cull;           /* culling box in world space */
 };
 
@@ -211,14 +212,26 @@ Box plane_box(const Plane &p, bool tri) {
   return b;
 }
 
-void write_record(std::vector<float> &out, const FlatPrim &p) {
-  uint32_t hdr = p.type | (p.xform << 2);
+float bits_f(uint32_t u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+/* ext: the side store of media boundaries */
+void write_record(std::vector<float> &out, const FlatPrim &p, const std::vector<float> &ext) {
+  uint32_t hdr = p.type | (p.xform << PT_XF_SHIFT);
   float fh, fi;
   memcpy(&fh, &hdr, 4);
   memcpy(&fi, &p.info, 4);
   const float *d = p.d;
   float z = 0.0f;
-  if (p.type == PT_SPHERE) {        /* [c, hdr] [r, 0, 0, info] */
+  if (p.type == PT_MEDIUM) {        /* [-1/density, kind, units, hdr] [0, 0, 0, info] boundary... */
+    const uint32_t kind = (uint32_t)d[1], units = 2u + p.ext_n / 4u;
+    float r[8] = {d[0], bits_f(kind), bits_f(units), fh, z, z, z, fi};
+    out.insert(out.end(), r, r + 8);
+    out.insert(out.end(), ext.begin() + p.ext, ext.begin() + p.ext + p.ext_n);
+  } else if (p.type == PT_SPHERE) { /* [c, hdr] [r, 0, 0, info] */
     float r[8] = {d[0], d[1], d[2], fh, d[3], z, z, fi};
     out.insert(out.end(), r, r + 8);
   } else if (p.type == PT_MSPHERE) { /* [c0, hdr] [r, 0, 0, info] [c1 - c0, 0] */
@@ -360,14 +373,15 @@ struct BvhBuilder {
 struct Collapser {
   const BvhBuilder &b;
   const std::vector<FlatPrim> &prims;
+  const std::vector<float> &ext;
   std::vector<Bvh4Node> out;
   std::vector<float> records;
   uint32_t depth = 0;
-  Collapser(const BvhBuilder &bb, const std::vector<FlatPrim> &p) : b(bb), prims(p) {}
+  Collapser(const BvhBuilder &bb, const std::vector<FlatPrim> &p, const std::vector<float> &e) : b(bb), prims(p), ext(e) {}
 
   uint32_t leaf_word(const BNode &n) {
     uint32_t off = (uint32_t)(records.size() / 4);
-    for (uint32_t i = 0; i < n.count; i++) write_record(records, prims[b.perm[n.first + i]]);
+    for (uint32_t i = 0; i < n.count; i++) write_record(records, prims[b.perm[n.first + i]], ext);
     return CHILD_LEAF | ((n.count - 1u) << 28) | off;
   }
   uint32_t collapse(int bi, uint32_t d) {
@@ -715,6 +729,17 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
         if (checker) use_checker();
         dm.mclass = m.kind == CLV_MAT_LAMBERTIAN ? MC_LAMBERTIAN : MC_METAL;
         break;
+      case CLV_MAT_ISOTROPIC:
+        /* isotropic.rs:21-44: albedo texture, diffuse scatter with the uniform-sphere pdf; shaded
+         * in the Lambertian queue.  A medium hit has no surface coordinates here. */
+        if (m.texture == CLV_TEX_SURFACE_CHECKER) {
+          *err = "a SurfaceChecker texture on a constant medium is not supported";
+          return CLV_EUNSUPPORTED;
+        }
+        dm.albedo_spd = plain_color ? spd_of(m.color) : illum_of(m.texture, m.intensity, true);
+        if (checker) use_checker();
+        dm.mclass = MC_LAMBERTIAN;
+        break;
       case CLV_MAT_DIELECTRIC:
         dm.albedo_spd = spd_of(m.color); /* spectral_power(color, lambda), dielectric.rs:79-81 */
         dm.mclass = MC_GLASS;
@@ -769,6 +794,8 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
 
   /* flatten objects */
   std::vector<FlatPrim> prims;
+  std::vector<float> ext;    /* media boundaries (float4 units) */
+  std::vector<float> media;  /* media records */
   prims.reserve(d->n_objects);
   std::vector<LItem> litems;
   for (uint32_t i = 0; i < d->n_objects; i++) {
@@ -867,6 +894,51 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
         local.push_back(fp);
         break;
       }
+      case CLV_OBJ_CONSTANT_MEDIUM_SPHERE:
+      case CLV_OBJ_CONSTANT_MEDIUM_BOXY: {       /* constant_medium.rs:51-63: -1/density, Isotropic(texture) */
+        if (o.priority) {
+          *err = "a ConstantMedium cannot be a light-sampling target (the reference's random() is unimplemented)";
+          return CLV_EINVAL;
+        }
+        if (d->materials[o.material].kind != CLV_MAT_ISOTROPIC) {
+          *err = "a ConstantMedium's material must be its Isotropic phase function";
+          return CLV_EINVAL;
+        }
+        const bool sph = o.kind == CLV_OBJ_CONSTANT_MEDIUM_SPHERE;
+        FlatPrim fp;
+        memset(&fp, 0, sizeof(fp));
+        fp.type = PT_MEDIUM;
+        fp.d[0] = -1.0f / p[sph ? 4 : 6];
+        fp.d[1] = (float)(sph ? MED_SPHERE : MED_BOX);
+        fp.ext = (uint32_t)ext.size();
+        if (sph) {
+          F3 c = f3(p[0], p[1], p[2]);
+          float r = p[3];
+          const float b[4] = {c.x, c.y, c.z, r};
+          ext.insert(ext.end(), b, b + 4);
+          obox = box_from(c - f3(r, r, r), c + f3(r, r, r));
+        } else {                                 /* the boundary Boxy's quads, boxy.rs:43-83 */
+          F3 c0 = f3(p[0], p[1], p[2]), c1 = f3(p[3], p[4], p[5]);
+          F3 mn = f3(fminf(c0.x, c1.x), fminf(c0.y, c1.y), fminf(c0.z, c1.z));
+          F3 mx = f3(fmaxf(c0.x, c1.x), fmaxf(c0.y, c1.y), fmaxf(c0.z, c1.z));
+          F3 dx = f3(mx.x - mn.x, 0, 0), dy = f3(0, mx.y - mn.y, 0), dz = f3(0, 0, mx.z - mn.z);
+          F3 q[6] = {f3(mn.x, mn.y, mx.z), f3(mx.x, mn.y, mx.z), f3(mx.x, mn.y, mn.z),
+                     f3(mn.x, mn.y, mn.z), f3(mn.x, mx.y, mx.z), f3(mn.x, mn.y, mn.z)};
+          F3 uu[6] = {dx, -dz, -dx, dz, dx, dx};
+          F3 vv[6] = {dy, dy, dy, dy, -dz, dz};
+          for (int k = 0; k < 6; k++) {
+            FlatPrim side;
+            memset(&side, 0, sizeof(side));
+            side.type = PT_QUAD;
+            quad_payload(&side, make_plane(q[k], uu[k], vv[k], false));
+            write_record(ext, side, ext);
+          }
+          obox = box_from(c0, c1);
+        }
+        fp.ext_n = (uint32_t)ext.size() - fp.ext;
+        local.push_back(fp);
+        break;
+      }
       case CLV_OBJ_BOXY: {                      /* boxy.rs:43-83 */
         F3 c0 = f3(p[0], p[1], p[2]), c1 = f3(p[3], p[4], p[5]);
         F3 mn = f3(fminf(c0.x, c1.x), fminf(c0.y, c1.y), fminf(c0.z, c1.z));
@@ -927,7 +999,12 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
       pi.material = o.material;
       pi.report_id = i * 8u + (uint32_t)k;
       hs->prim_info.push_back(pi);
-      prims.push_back(fp);
+      if (fp.type == PT_MEDIUM) {  /* media stay out of the tree: their records follow the leaves' */
+        write_record(media, fp, ext);
+        hs->n_media++;
+      } else {
+        prims.push_back(fp);
+      }
     }
   }
   if (litems.empty()) {
@@ -948,7 +1025,7 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
     }
     for (size_t i = 0; i < prims.size(); i++) {
       hs->prim_unit[i] = (uint32_t)(hs->prims.size() / 4);
-      write_record(hs->prims, prims[i]);
+      write_record(hs->prims, prims[i], ext);
       const Box &b = prims[i].cull;
       float *o = &hs->prim_box[8 * i];
       for (int a = 0; a < 3; a++) {
@@ -966,7 +1043,7 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
   } else {
     BvhBuilder bb(prims);
     int root = bb.build(0, (uint32_t)prims.size(), 0);
-    Collapser col(bb, prims);
+    Collapser col(bb, prims, ext);
     col.out.reserve(bb.nodes.size() / 2 + 1);
     col.records.reserve(prims.size() * 16);
     col.collapse(root, 0);
@@ -978,6 +1055,8 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
     if (3u * (col.depth + 1u) > 72u) { *err = "scene BVH deeper than the traversal stack allows"; return CLV_EUNSUPPORTED; }
     if (hs->prims.size() / 4 > LEAF_OFFSET_MASK) { *err = "scene too large for 28-bit record offsets"; return CLV_EUNSUPPORTED; }
   }
+  hs->media_units = (uint32_t)(media.size() / 4);
+  hs->prims.insert(hs->prims.end(), media.begin(), media.end());
 
   /* light tree */
   LightTreeBuilder lt;
@@ -1011,15 +1090,16 @@ int rebase_host_scene(HostScene *hs, const ArenaBase &b, std::string *err) {
     }
   float *rec = hs->prims.data();
   for (uint64_t u = 0; u < units;) {
-    uint32_t hdr, info;
+    uint32_t hdr, info, size;
     memcpy(&hdr, rec + 4 * u + 3, 4);
     memcpy(&info, rec + 4 * (u + 1) + 3, 4);
-    const uint32_t type = hdr & 3u, xf = hdr >> 2;
-    if (xf) hdr = type | ((xf + b.xforms) << 2);
+    memcpy(&size, rec + 4 * u + 2, 4);
+    const uint32_t type = hdr & PT_TYPE_MASK, xf = hdr >> PT_XF_SHIFT;
+    if (xf) hdr = type | ((xf + b.xforms) << PT_XF_SHIFT);
     info += b.info;
     memcpy(rec + 4 * u + 3, &hdr, 4);
     memcpy(rec + 4 * (u + 1) + 3, &info, 4);
-    u += PREC_OF(type);
+    u += PREC_UNITS(type, size);
   }
   for (PrimInfo &pi : hs->prim_info) pi.material += b.mats;
   for (DevMaterial &m : hs->materials) {

@@ -27,9 +27,12 @@ struct HostScene {
   uint32_t n_prims = 0;
   uint32_t bvh_depth = 0;
   uint32_t n_objects = 0;
+  uint32_t n_media = 0;               /* constant media: their records end prims, media_units long */
+  uint32_t media_units = 0;
   double build_ms = 0.0;
   /* built for the device builder: no nodes; prims holds the records in
-   * primitive order, with each one's offset (float4 units) and culling box */
+   * primitive order (then the media), with each one's offset (float4 units)
+   * and culling box */
   bool device_bvh = false;
   std::vector<uint32_t> prim_unit;
   std::vector<float> prim_box;        /* 8 floats per primitive: lo.xyz, 0, hi.xyz, 0 */

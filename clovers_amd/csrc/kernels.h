@@ -41,7 +41,8 @@ struct DevScene {
   const float *lweights;
   const DevJob *jobs;
   uint32_t has_xforms;    /* some scene of the batch has instance chains */
-  uint32_t has_ext;       /* some scene of the batch has checker textures or cone lights */
+  uint32_t has_ext;       /* some scene of the batch has checker textures, cone lights or media */
+  uint32_t has_media;     /* some scene of the batch has constant media */
 };
 
 /* Structure-of-arrays path state.  A pass has one slot per sample (a pixel

@@ -74,6 +74,14 @@ __device__ __forceinline__ uint64_t pcg_for_path(uint64_t seed, uint32_t pixel, 
 __device__ __forceinline__ float rnd(uint64_t &s) { return (float)(pcg_next(s) >> 8) * 0x1p-24f; }
 __device__ __forceinline__ bool coin(uint64_t &s) { return (pcg_next(s) >> 31) != 0u; }
 __device__ __forceinline__ float rnd_pm1(uint64_t &s) { return 2.0f * rnd(s) - 1.0f; }
+/* the free-flight uniform of a constant medium, in [0, 1) like rnd: a hash of
+ * the path's stream state at the bounce and the medium's object id (the
+ * reference draws it from the path's generator inside the traversal, in its
+ * own tree's order, constant_medium.rs:103) */
+__device__ __forceinline__ float medium_u01(uint64_t state, uint32_t id) {
+  const uint64_t h = splitmix64(state ^ (((uint64_t)id + 1ull) * 0xD1B54A32D192ED03ull));
+  return (float)(uint32_t)(h >> 40) * 0x1p-24f;
+}
 
 /* wave-aggregated append of `item` to queue q (counter *cnt) when pred */
 __device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *q, uint32_t *cnt) {
@@ -225,14 +233,60 @@ __device__ __forceinline__ void xform_ray(const Xform &x, V &o, V &d) {
   }
 }
 
-/* test one primitive record; on a hit inside [tmin, tbest] update tbest */
+/* Boxy::hit (boxy.rs:87-104) over its six quad records: the closest side */
+__device__ __forceinline__ bool box_t(const float4 *sides, V o, V d, float tmin, float tmax, float &t) {
+  bool any = false;
+  float closest = tmax;
+  for (int k = 0; k < 6; k++) {
+    float tt;
+    if (quad_t(sides + PREC_QUAD * k, o, d, tmin, closest, tt)) {
+      closest = tt;
+      any = true;
+    }
+  }
+  t = closest;
+  return any;
+}
+
+#define EPS_MEDIUM 0.0001f /* EPSILON_CONSTANT_MEDIUM, lib.rs:119 */
+
+/* ConstantMedium::hit (constant_medium.rs:65-126): the boundary's entry
+ * over (-inf, inf) and exit after it, clipped to [tmin, tmax] and to 0, an
+ * exponential free flight of mean 1/density along the ray's length */
+__device__ __forceinline__ bool medium_t(const DevScene &sc, const float4 *rec, V o, V d, float tmin, float tmax,
+                                         uint64_t state, float &t) {
+  const float4 f0 = rec[0];
+  float t1, t2;
+  if (as_u(f0.y) == MED_SPHERE) {
+    const float4 b = rec[2];
+    if (!sphere_t(xyz(b), b.w, o, d, -INFINITY, INFINITY, t1)) return false;
+    if (!sphere_t(xyz(b), b.w, o, d, t1 + EPS_MEDIUM, INFINITY, t2)) return false;
+  } else {
+    if (!box_t(rec + 2, o, d, -INFINITY, INFINITY, t1)) return false;
+    if (!box_t(rec + 2, o, d, t1 + EPS_MEDIUM, INFINITY, t2)) return false;
+  }
+  if (t1 < tmin) t1 = tmin;
+  if (t2 > tmax) t2 = tmax;
+  if (t1 >= t2) return false;
+  if (t1 < 0.0f) t1 = 0.0f;
+  const float len = sqrtf(dot(d, d));
+  const float inside = (t2 - t1) * len;
+  const float u = medium_u01(state, sc.info[as_u(rec[1].w)].report_id);
+  const float hd = f0.x * logf(u);
+  if (hd > inside) return false;
+  t = t1 + hd / len;
+  return true;
+}
+
+/* test one primitive record of a BVH leaf (media are not in the tree); on a
+ * hit inside [tmin, tbest] update tbest */
 __device__ __forceinline__ uint32_t prim_units(uint32_t type) { return PREC_OF(type); }
 
 template <bool HAS_XFORM>
 __device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec, uint32_t hdr, V o, V d, const WRay &wr,
                                           float time, float tmin, float tmax, float &t) {
-  uint32_t type = hdr & 3u;
-  uint32_t xf = hdr >> 2;
+  uint32_t type = hdr & PT_TYPE_MASK;
+  uint32_t xf = hdr >> PT_XF_SHIFT;
   WRay w2 = wr;
   if (HAS_XFORM && xf) {
     xform_ray(sc.xforms[xf - 1u], o, d);
@@ -363,9 +417,33 @@ __device__ __forceinline__ uint32_t lane_leaf(Lane &L, const DevScene &sc, bool 
       L.tb = t;
       L.best = off;
     }
-    off += prim_units(hdr & 3u);
+    off += prim_units(hdr & PT_TYPE_MASK);
   }
   return cnt;
+}
+
+/* After the surface traversal: the job's constant media, each tested against
+ * the closest surface so far (a medium's hit needs its boundary inside
+ * [tmin, tmax], so the tree's box culling never changes its outcome, and its
+ * uniform is keyed by the medium, so the order of the tests does not matter).
+ * Only the traversal instance for batches with media runs this. */
+__device__ __forceinline__ void lane_media(Lane &L, const DevScene &sc, const PathBuf &pb) {
+  const DevJob &jb = sc.jobs[pb.meta[L.path] >> META_JOB_SHIFT];
+  if (jb.n_media == 0) return;
+  const uint64_t state = pb.rng[L.path];
+  uint32_t off = jb.media_off;
+  for (uint32_t k = 0; k < jb.n_media; k++) {
+    const float4 *rec = sc.prims + off;
+    const uint32_t hdr = as_u(rec[0].w), xf = hdr >> PT_XF_SHIFT;
+    V o = L.o, d = L.d;
+    if (xf) xform_ray(sc.xforms[xf - 1u], o, d);
+    float t;
+    if (medium_t(sc, rec, o, d, EPS_ACNE, L.tb, state, t)) {
+      L.tb = t;
+      L.best = off;
+    }
+    off += as_u(rec[0].z);
+  }
 }
 
 /* ------------------------------------------------------------ kernels --- */
@@ -527,7 +605,7 @@ __device__ __forceinline__ void lprim_at(Lane &L, const DevScene &sc, const floa
     L.tb = t;
     L.best = off;
   }
-  units = prim_units(hdr & 3u);
+  units = prim_units(hdr & PT_TYPE_MASK);
 }
 
 template <bool COUNT, int MODE>
@@ -604,7 +682,10 @@ __global__ __launch_bounds__(LT_BLOCK) void k_trace_lds(DevScene sc, PathBuf pb,
 /* Closest-hit kernel, one ray per thread: the block's threads take rays
  * base + threadIdx.x in a grid-stride loop; each thread runs its ray to
  * completion (node or leaf per step).  Results as in k_trace below. */
-template <bool COUNT, int MODE, bool WW>
+/* MEDIA: the instance for batches with constant media (the media pass after
+ * the tree raises the traversal's registers, so the other instances leave it
+ * out; batches with media always trace with this kernel) */
+template <bool COUNT, int MODE, bool WW, bool MEDIA = false>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathBuf pb, TraceArgs ta) {
   __shared__ uint32_t lds_stack[STACK_LDS * TRACE_BLOCK];
   __shared__ uint32_t agg[MC_COUNT * (TRACE_BLOCK / 64) + MC_COUNT];
@@ -651,6 +732,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_simple(DevScene sc, PathB
         }
       }
     }
+    if (MEDIA && active) lane_media(L, sc, pb);
     if (MODE == 0) {
       uint32_t cls = MC_COUNT;
       if (active) {
@@ -794,7 +876,7 @@ template <bool UV>
 __device__ Hit hit_record(const DevScene &sc, uint32_t off, V o, V d, float time, float t) {
   const float4 *rec = sc.prims + off;
   float4 f0 = rec[0], f1 = rec[1];
-  uint32_t hdr = as_u(f0.w), type = hdr & 3u, xf = hdr >> 2;
+  uint32_t hdr = as_u(f0.w), type = hdr & PT_TYPE_MASK, xf = hdr >> PT_XF_SHIFT;
   Hit h;
   h.material = sc.info[as_u(f1.w)].material;
   h.u = h.v = 0.0f;
@@ -822,7 +904,12 @@ __device__ Hit hit_record(const DevScene &sc, uint32_t off, V o, V d, float time
   V p = o + t * d;
   V outward;
   const bool want_uv = UV && sc.mats[h.material].tex == TX_SURFACE;
-  if (type == PT_SPHERE || type == PT_MSPHERE) {
+  if (type == PT_MEDIUM) {
+    /* the reference gives a medium hit a random unit normal and front_face = true
+     * (constant_medium.rs:108-109); nothing reads them for an Isotropic phase
+     * function, so a fixed one is used and the stream is not drawn from */
+    outward = -unit(d);
+  } else if (type == PT_SPHERE || type == PT_MSPHERE) {
     V c = type == PT_SPHERE ? xyz(f0) : xyz(f0) + time * xyz(rec[2]);
     outward = unit((p - c) / f1.x);
     if (want_uv) sphere_uv(p, c, f1.x, h.u, h.v);
@@ -1095,7 +1182,22 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, const PathBuf &pb
     }
   }
   V nd;
-  if (CLS == MC_LAMBERTIAN) {
+  if (CLS == MC_LAMBERTIAN && EXT && m.kind == CLV_MAT_ISOTROPIC) {
+    /* MixturePDF(HitablePDF(lights), SpherePDF) -- isotropic.rs:21-39, pdf.rs:144-168; never guided,
+     * no training record (a medium vertex counts like a specular one for the credit rule) */
+    if (GUIDE && ga.record && ga.indirect && jb.guiding) {
+      const uint32_t g = pb.gcnt[slot];
+      if (g && !(g & GD_SPECULAR)) pb.gcnt[slot] = g | GD_SPECULAR;
+    }
+    V gen = coin(s) ? tree_sample(sc, jb, h.p, s) : random_unit_vector(s);
+    nd = unit(gen);
+    const float sp = 1.0f / (4.0f * PI_F);
+    const float mis = 0.5f * tree_pdf(sc, jb, h.p, nd) + 0.5f * sp;
+    T.x = T.x * att[0] * sp / mis;
+    T.y = T.y * att[1] * sp / mis;
+    T.z = T.z * att[2] * sp / mis;
+    T.w = T.w * att[3] * sp / mis;
+  } else if (CLS == MC_LAMBERTIAN) {
     /* MixturePDF(HitablePDF(lights), CosinePDF(normal)) -- trace.rs:80-120, pdf.rs:106-141 */
     Onb b = onb_from_w(h.n);
     /* guided: the light-tree half of the mixture stays; the cosine half draws
@@ -1372,7 +1474,8 @@ __global__ __launch_bounds__(256) void k_generate_debug(DevScene sc, PathBuf pb,
     dir = unit(dir);
     pb.ray_o[i] = make_float4(org.x, org.y, org.z, __uint_as_float(jb.root));
     pb.ray_d[i] = make_float4(dir.x, dir.y, dir.z, time);
-    pb.meta[i] = wl;
+    pb.meta[i] = wl;   /* job 0 */
+    pb.rng[i] = s;     /* the stream state a medium's free flight is keyed by */
   }
 }
 
@@ -1535,6 +1638,18 @@ uint32_t lds_trace_grid_blocks(size_t lds_bytes) {
 
 void launch_trace_lds(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                       size_t lds_bytes, bool counting, int mode) {
+  if (sc.has_media) { /* the host plans media batches with the plain traversal; this only keeps them correct */
+    if (mode == 1)
+      hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc,
+                         pb, ta);
+    else if (counting)
+      hipLaunchKernelGGL((k_trace_simple<true, MODE_EXTEND, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb,
+                         ta);
+    else
+      hipLaunchKernelGGL((k_trace_simple<false, MODE_EXTEND, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc,
+                         pb, ta);
+    return;
+  }
   if (mode == 1)
     hipLaunchKernelGGL((k_trace_lds<false, 1>), dim3(blocks), dim3(LT_BLOCK), lds_bytes, st, sc, pb, ta);
   else if (counting)
@@ -1545,7 +1660,14 @@ void launch_trace_lds(hipStream_t st, const DevScene &sc, const PathBuf &pb, con
 
 void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                    bool counting, int variant) {
-  if (variant == TRACE_PERSISTENT) {
+  if (sc.has_media) {
+    if (counting)
+      hipLaunchKernelGGL((k_trace_simple<true, MODE_EXTEND, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb,
+                         ta);
+    else
+      hipLaunchKernelGGL((k_trace_simple<false, MODE_EXTEND, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc,
+                         pb, ta);
+  } else if (variant == TRACE_PERSISTENT) {
     if (counting)
       hipLaunchKernelGGL((k_trace<true, MODE_EXTEND>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
     else
@@ -1565,7 +1687,10 @@ void launch_extend(hipStream_t st, const DevScene &sc, const PathBuf &pb, const 
 
 void launch_primary(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta, uint32_t blocks,
                     int variant) {
-  if (variant == TRACE_PERSISTENT)
+  if (sc.has_media)
+    hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb,
+                       ta);
+  else if (variant == TRACE_PERSISTENT)
     hipLaunchKernelGGL((k_trace<false, MODE_PRIMARY>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
   else
     hipLaunchKernelGGL((k_trace_simple<false, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
@@ -1575,7 +1700,11 @@ void launch_generate_debug(hipStream_t st, const DevScene &sc, const PathBuf &pb
 }
 void launch_primary_counting(hipStream_t st, const DevScene &sc, const PathBuf &pb, const TraceArgs &ta,
                              uint32_t blocks) {
-  hipLaunchKernelGGL((k_trace_simple<true, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
+  if (sc.has_media)
+    hipLaunchKernelGGL((k_trace_simple<true, MODE_PRIMARY, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb,
+                       ta);
+  else
+    hipLaunchKernelGGL((k_trace_simple<true, MODE_PRIMARY, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, sc, pb, ta);
 }
 void launch_debug_color(hipStream_t st, const DevScene &sc, const PathBuf &pb, const DebugArgs &da,
                         const uint32_t *ray_counts, float *out) {

@@ -14,8 +14,9 @@
  * f32 in the reference's operation order (compiled with -ffp-contract=off).
  * Colour conversions (palette 0.7.6) are evaluated in f64 and rounded once
  * to f32.  What the reference's schema allows but this build does not render
- * (ConstantMedium, GLTF, Isotropic, other illuminants and colour spaces) is
- * CLV_EUNSUPPORTED; what makes the reference panic is CLV_EINVAL.
+ * (GLTF, ConstantMedium boundaries other than Sphere/Boxy, other illuminants
+ * and colour spaces) is CLV_EUNSUPPORTED; what makes the reference panic is
+ * CLV_EINVAL.
  */
 #include "../../include/clovers_hip.h"
 
@@ -414,6 +415,10 @@ clv_material material(const J *m) {
     r.kind = CLV_MAT_CONE_LIGHT;
     r.spread = (float)num(m->get("spread"), "spread");
     texture(m->get("emit"), bright, r);
+  } else if (k == "Isotropic") {  /* isotropic.rs:13-19: albedo texture, default grey */
+    r.kind = CLV_MAT_ISOTROPIC;
+    texture(m->get("albedo"), grey, r);
+    if (r.texture == CLV_TEX_SURFACE_CHECKER) unsupported("a SurfaceChecker texture on an Isotropic material");
   } else {
     unsupported("material kind '" + k + "' is not supported");
   }
@@ -742,6 +747,53 @@ struct Loader {
     uint32_t mat = material_of(o.get("material"), false);
     push(kind, p, np, mat, priority, add_xform(ops));
   }
+  /* ConstantMedium (objects.rs:99-103, constant_medium.rs:19-46): a Sphere or
+   * Boxy boundary (its own wrappers move onto the medium: the same hit along
+   * the same ray), density (default 0.1), texture (default SolidColor grey)
+   * as the Isotropic phase function; the boundary's material is never read */
+  void medium(const J &o, bool priority, std::vector<clv_xform_op> ops) {
+    if (priority) invalid("a ConstantMedium cannot be a priority object (its random() is unimplemented: the reference panics)");
+    const J *b = &need(o, "boundary");
+    for (;;) {
+      const std::string bk = kind_of(*b);
+      if (bk != "Translate" && bk != "RotateY") break;
+      clv_xform_op op;
+      memset(&op, 0, sizeof(op));
+      if (bk == "Translate") {
+        op.kind = CLV_XF_TRANSLATE;
+        vec3f(need(*b, "offset"), op.v);
+      } else {
+        op.kind = CLV_XF_ROTATE_Y;
+        op.v[0] = (float)num(b->get("angle"), "angle");
+      }
+      ops.push_back(op);
+      b = &need(*b, "object");
+    }
+    const std::string bk = kind_of(*b);
+    const float density = (float)num_or(o, "density", 0.1);
+    float p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t kind;
+    int np;
+    if (bk == "Sphere") {
+      vec3f(need(*b, "center"), p);
+      p[3] = (float)num(b->get("radius"), "radius");
+      p[4] = density;
+      kind = CLV_OBJ_CONSTANT_MEDIUM_SPHERE, np = 5;
+    } else if (bk == "Boxy") {
+      vec3f(need(*b, "corner_0"), p);
+      vec3f(need(*b, "corner_1"), p + 3);
+      p[6] = density;
+      kind = CLV_OBJ_CONSTANT_MEDIUM_BOXY, np = 7;
+    } else {
+      unsupported("a ConstantMedium boundary of kind '" + bk + "' is not supported");
+    }
+    clv_material iso = material(nullptr);  /* the defaults of every material field */
+    iso.kind = CLV_MAT_ISOTROPIC;
+    const float grey[3] = {0.5f, 0.5f, 0.5f};
+    texture(o.get("texture"), grey, iso);
+    if (iso.texture == CLV_TEX_SURFACE_CHECKER) unsupported("a SurfaceChecker texture on a ConstantMedium");
+    push(kind, p, np, add_material(iso), false, add_xform(ops));
+  }
   void mesh(const J &o, bool priority, bool ply) {
     std::string path = need(o, "path").t == J::STR ? need(o, "path").s : "";
     if (path.empty()) invalid("mesh without a path");
@@ -784,6 +836,10 @@ struct Loader {
       const J &l = need(o, "objects");
       if (l.t != J::ARR) invalid("ObjectList objects must be a list");
       for (const J &c : l.a) add(c, priority, {}, false);  /* flattened one level (scenefile.rs:84-94) */
+      return;
+    }
+    if (k == "ConstantMedium") {
+      medium(o, priority, ops);
       return;
     }
     primitive(o, priority, ops);

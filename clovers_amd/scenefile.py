@@ -10,8 +10,9 @@ internally tagged ``kind`` for objects, materials and textures
 (a shared-material name or an inline material, materials.rs:33-39) and
 untagged ColorInit (colorinit.rs:17-43).
 
-Out of scope (raise NotImplementedError): GLTF files, constant media,
-isotropic, and the illuminants other than D65/D50.  A list nested inside
+Out of scope (raise NotImplementedError): GLTF files, constant media with a
+boundary other than a Sphere or a Boxy, and the illuminants other than
+D65/D50.  A list nested inside
 RotateY/Translate is rejected too: the reference's HitableList::hit returns
 None as soon as any child misses (hitable.rs:158), which no scene relies on.
 STL and PLY meshes are read by clovers_amd.meshfile.
@@ -189,6 +190,11 @@ def material_record(m):
         r["kind"] = abi.MAT_DIFFUSE_LIGHT
         # DiffuseLight::default(): SolidColor Xyz(100,100,100) (diffuse_light.rs:23-30)
         _set(r, _texture(m.get("emit"), (100.0, 100.0, 100.0)))
+    elif kind == "Isotropic":   # isotropic.rs:13-19: albedo texture, default grey
+        r["kind"] = abi.MAT_ISOTROPIC
+        _set(r, _texture(m.get("albedo"), (0.5, 0.5, 0.5)))
+        if r["texture"] == abi.TEX_SURFACE_CHECKER:
+            raise NotImplementedError("a SurfaceChecker texture on an Isotropic material")
     elif kind == "ConeLight":
         # both fields are required by the scene-file schema (cone_light.rs:13-18: no serde defaults)
         if "spread" not in m or "emit" not in m:
@@ -256,9 +262,39 @@ class _Loader:
             if ops:
                 raise NotImplementedError("a list inside RotateY/Translate (hitable.rs:158 returns None on any miss)")
             return [("list" if k == "ObjectList" else "mesh", o)]
+        if k == "ConstantMedium":
+            return [("medium", o, ops)]
         if k not in _PRIM_KINDS:
             raise NotImplementedError("object kind %r is not supported" % k)
         return [("prim", o, ops)]
+
+    def medium(self, o, priority, ops):
+        """ConstantMedium (objects.rs:99-103, constant_medium.rs:19-46): a
+        Sphere or Boxy boundary, whose own RotateY/Translate wrappers move onto
+        the medium (the same hit along the same ray); density (default 0.1);
+        texture (default SolidColor grey) as the Isotropic phase function.  The
+        boundary's material is never read."""
+        if priority:
+            raise ValueError("a ConstantMedium cannot be a priority object (its random() is unimplemented: "
+                             "the reference panics)")
+        b = o["boundary"]
+        ops = list(ops)
+        while b["kind"] in ("Translate", "RotateY"):
+            if b["kind"] == "Translate":
+                ops.append((abi.XF_TRANSLATE, list(b["offset"])))
+            else:
+                ops.append((abi.XF_ROTATE_Y, [float(b["angle"]), 0.0, 0.0]))
+            b = b["object"]
+        density = float(o.get("density", 0.1))
+        if b["kind"] == "Sphere":
+            kind, params = abi.OBJ_MEDIUM_SPHERE, list(b["center"]) + [b["radius"], density]
+        elif b["kind"] == "Boxy":
+            kind, params = abi.OBJ_MEDIUM_BOXY, list(b["corner_0"]) + list(b["corner_1"]) + [density]
+        else:
+            raise NotImplementedError("a ConstantMedium boundary of kind %r is not supported" % b["kind"])
+        r = material_record({"kind": "Isotropic", "albedo": o.get("texture")})
+        mat = self.scene.add_material(r)
+        self.scene.add_object(kind, params, mat, False, self.scene.add_xform(ops))
 
     def mesh_material(self, init):
         """STL/PLY material: a shared name must exist (stl.rs:50, ply.rs:43
@@ -287,6 +323,9 @@ class _Loader:
                 tris = mesh_triangles(entry[1], self.mesh_root)
                 mat = self.mesh_material(entry[1].get("material"))
                 self.scene.add_objects(abi.OBJ_TRIANGLE, tris.reshape(-1, 9), mat, priority)
+                continue
+            if entry[0] == "medium":
+                self.medium(entry[1], priority, entry[2])
                 continue
             _, prim, ops = entry
             kind, params = _leaf_params(prim)

@@ -175,6 +175,48 @@ def features():
     return load_scene(features_json())
 
 
+def media_json():
+    """Constant media in a lit room (objects/constant_medium.rs, the smoke
+    boxes of the reference's cornell_smoke / the_next_week scenes): a fog
+    sphere of grey Isotropic albedo, a smoke box with a spatial-checker
+    albedo under Translate{RotateY} with its own wrapper inside the boundary,
+    a thin blue haze around everything, a Lambertian floor and a lamp."""
+    mats = [
+        {"name": "floor", "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": [0.7, 0.7, 0.7]}},
+        {"name": "lamp", "kind": "DiffuseLight", "emit": {"kind": "IlluminantD65", "intensity": 0.15}},
+        {"name": "ball", "kind": "Lambertian", "albedo": {"kind": "SolidColor", "color": [0.8, 0.3, 0.2]}},
+    ]
+    objs = [
+        {"kind": "Quad", "q": [-5, 0, -5], "u": [10, 0, 0], "v": [0, 0, 10], "material": "floor"},
+        {"kind": "Quad", "priority": True, "q": [-1, 3.99, -1], "u": [2, 0, 0], "v": [0, 0, 2], "material": "lamp"},
+        {"kind": "Sphere", "center": [1.6, 0.5, 0.8], "radius": 0.5, "material": "ball"},
+        {"kind": "ConstantMedium", "density": 1.2,
+         "texture": {"kind": "SolidColor", "color": [0.9, 0.9, 0.9]},
+         "boundary": {"kind": "Sphere", "center": [-1.2, 1.0, 0.0], "radius": 0.9}},
+        {"kind": "Translate", "offset": [0.6, 0.0, -1.2], "object":
+            {"kind": "ConstantMedium", "density": 2.5,
+             "texture": {"kind": "SpatialChecker", "even": {"xyz_e": [0.9, 0.8, 0.6]}, "odd": {"xyz_e": [0.2, 0.2, 0.3]},
+                         "density": 2.0},
+             "boundary": {"kind": "RotateY", "angle": 30, "object":
+                          {"kind": "Boxy", "corner_0": [0, 0, 0], "corner_1": [1.0, 1.8, 1.0]}}}},
+        {"kind": "ConstantMedium", "density": 0.02,
+         "texture": {"kind": "SolidColor", "color": [0.6, 0.7, 1.0]},
+         "boundary": {"kind": "Sphere", "center": [0.0, 0.0, 0.0], "radius": 20.0}},
+    ]
+    return {
+        "time_0": 0, "time_1": 1,
+        "camera": {"look_from": [0.0, 2.0, 7.0], "look_at": [0.0, 1.0, 0.0], "up": [0.0, 1.0, 0.0],
+                   "vertical_fov": 45.0, "aperture": 0.0, "focus_distance": 7.0},
+        "background_color": [0.1, 0.1, 0.12],
+        "objects": objs,
+        "materials": mats,
+    }
+
+
+def media():
+    return load_scene(media_json())
+
+
 # ------------------------------------------------------------------- prism
 def prism_triangles(center=(0.0, 0.55, 0.0), side=0.8, length=1.2, yaw_deg=30.0):
     """12 triangles: an equilateral prism along z whose two caps are

@@ -51,10 +51,19 @@ typedef enum {
   CLV_OBJ_TRIANGLE = 2, /* p = a.xyz, b.xyz, c.xyz (absolute corners)     (objects/triangle.rs:106-117) */
   CLV_OBJ_BOXY = 3,     /* p = corner_0.xyz, corner_1.xyz                  (objects/boxy.rs:43-83) */
   CLV_OBJ_TRIANGLE_QUV = 4, /* p = q.xyz, u.xyz, v.xyz (scene-file Triangle)  (objects/triangle.rs:61-101) */
-  CLV_OBJ_MOVING_SPHERE = 5 /* p = center_0.xyz, center_1.xyz, radius; time_0 = 0, time_1 = 1
+  CLV_OBJ_MOVING_SPHERE = 5, /* p = center_0.xyz, center_1.xyz, radius; time_0 = 0, time_1 = 1
                                (objects/moving_sphere.rs, objects.rs:104-109).  Never a light-sampling
                                target: its sampler is unimplemented in the reference (a panic), so
                                priority = 1 is CLV_EINVAL */
+  /* ConstantMedium (objects/constant_medium.rs:65-126, objects.rs:99-103) with a Sphere or a Boxy
+   * boundary; material = a CLV_MAT_ISOTROPIC phase function carrying the medium's texture.  A
+   * boundary's own RotateY/Translate wrappers go on the medium object (the same hit: distances are
+   * measured along the same ray).  Never a light-sampling target (random() is unimplemented in
+   * the reference), so priority = 1 is CLV_EINVAL.  The free-flight uniform that the reference
+   * draws from the path's generator during traversal is drawn from a hash of the path's stream
+   * state and the medium's object index instead, so that it does not depend on traversal order. */
+  CLV_OBJ_CONSTANT_MEDIUM_SPHERE = 6, /* p = center.xyz, radius, density */
+  CLV_OBJ_CONSTANT_MEDIUM_BOXY = 7    /* p = corner_0.xyz, corner_1.xyz, density */
 } clv_object_kind;
 
 /* One top-level hitable of the reference's flattened list (scenefile.rs:63-103):
@@ -92,7 +101,9 @@ typedef enum {
   CLV_MAT_DIELECTRIC = 2,     /* materials/dielectric.rs  refractive_index, color (Xyz<E>) */
   CLV_MAT_DISPERSIVE = 3,     /* materials/dispersive.rs  cauchy_a, cauchy_b (um^2) */
   CLV_MAT_DIFFUSE_LIGHT = 4,  /* materials/diffuse_light.rs  emit texture */
-  CLV_MAT_CONE_LIGHT = 5      /* materials/cone_light.rs  emit texture, spread (degrees) */
+  CLV_MAT_CONE_LIGHT = 5,     /* materials/cone_light.rs  emit texture, spread (degrees) */
+  CLV_MAT_ISOTROPIC = 6       /* materials/isotropic.rs   albedo texture (not a surface checker); only
+                                 as the phase function of a constant medium */
 } clv_material_kind;
 
 /* Texture kinds: clovers/src/textures.rs:22-109 (in-scope subset). */

@@ -346,6 +346,7 @@ struct ScatterRecord {
   ScatterKind kind;
   Ray specular_ray;
   ONB cosine_uvw;       /* CosinePDF (pdf.rs:41-73); ZeroPDF for specular */
+  bool sphere = false;  /* SpherePDF (pdf.rs:144-168) instead of the cosine lobe: Isotropic */
 };
 
 struct Material {
@@ -402,7 +403,7 @@ struct Material {
     Float tf = thin_film ? interference(ray.direction, wl, rec) : 1.0f;
     Float k;
     switch (kind) {
-      case CLV_MAT_LAMBERTIAN: case CLV_MAT_METAL: k = tex_color(wl, rec); break;
+      case CLV_MAT_LAMBERTIAN: case CLV_MAT_METAL: case CLV_MAT_ISOTROPIC: k = tex_color(wl, rec); break;
       case CLV_MAT_DIELECTRIC: k = spd[wl - MIN_WAVELENGTH]; break; /* dielectric.rs:79-81 */
       case CLV_MAT_DISPERSIVE: k = 1.0f; break;                     /* dispersive.rs:121-123 */
       default: k = fminf(fmaxf(tex_color(wl, rec), 0.0f), 1.0f); break;  /* diffuse_light.rs:55-57, cone_light.rs:69-71 */
@@ -447,6 +448,11 @@ struct Material {
       case CLV_MAT_LAMBERTIAN:   /* lambertian.rs:22-34 */
         out->kind = DIFFUSE;
         out->cosine_uvw = ONB::from_w(rec.normal);
+        out->sphere = false;
+        return true;
+      case CLV_MAT_ISOTROPIC:    /* isotropic.rs:21-32: diffuse, SpherePDF, no draws */
+        out->kind = DIFFUSE;
+        out->sphere = true;
         return true;
       case CLV_MAT_METAL: {      /* metal.rs:26-52 */
         V3 reflected = reflect(ray.direction, rec.normal);
@@ -473,6 +479,10 @@ struct Material {
   }
   /* lambertian.rs:39-46; other kinds: None (materials.rs:120-122) */
   bool scattering_pdf(const HitRecord &rec, const Ray &scattered, Float *out) const {
+    if (kind == CLV_MAT_ISOTROPIC) {  /* isotropic.rs:35-37 */
+      *out = 1.0f / (4.0f * PI);
+      return true;
+    }
     if (kind != CLV_MAT_LAMBERTIAN) return false;
     Float cosine = dot(rec.normal, normalize(scattered.direction));
     if (cosine < 0.0f) return false;
@@ -791,6 +801,54 @@ struct Translate : Hitable {  /* objects/translate.rs */
   Hitable *clone() const override { return new Translate(*this); }
 };
 
+/* The stream state of the path being traced: a constant medium's free-flight
+ * uniform is a hash of it and the medium's object id, so that it does not
+ * depend on the order in which a traversal meets the medium (the reference
+ * draws it from the path's generator inside its own tree's traversal,
+ * constant_medium.rs:103).  Set by trace() and the probes before each
+ * traversal; one per thread. */
+static thread_local uint64_t medium_key = 0;
+static Float medium_u01(uint64_t state, uint32_t id) {
+  const uint64_t h = orc_splitmix64(state ^ (((uint64_t)id + 1ull) * 0xD1B54A32D192ED03ull));
+  return (Float)(uint32_t)(h >> 40) * 0x1p-24f;
+}
+
+struct ConstantMedium : Hitable {  /* objects/constant_medium.rs */
+  HPtr boundary; Float neg_inv_density; const Material *phase; uint32_t id;
+  ConstantMedium(HPtr b, Float density, const Material *m, uint32_t pid)
+      : boundary(std::move(b)), neg_inv_density(-1.0f / density), phase(m), id(pid) {}
+  bool hit(const Ray &ray, Float dmin, Float dmax, HitRecord *rec) const override { /* constant_medium.rs:65-126 */
+    HitRecord r1, r2;
+    if (!boundary->hit(ray, -INFINITY, INFINITY, &r1)) return false;
+    if (!boundary->hit(ray, r1.distance + 0.0001f, INFINITY, &r2)) return false;  /* EPSILON_CONSTANT_MEDIUM */
+    Float t1 = r1.distance, t2 = r2.distance;
+    if (t1 < dmin) t1 = dmin;
+    if (t2 > dmax) t2 = dmax;
+    if (t1 >= t2) return false;
+    if (t1 < 0.0f) t1 = 0.0f;
+    Float ray_length = magnitude(ray.direction);
+    Float inside = (t2 - t1) * ray_length;
+    Float hit_distance = neg_inv_density * logf(medium_u01(medium_key, id));
+    if (hit_distance > inside) return false;
+    Float distance = t1 + hit_distance / ray_length;
+    rec->distance = distance;
+    rec->position = ray.evaluate(distance);
+    /* a random unit normal and front_face = true in the reference; nothing
+     * reads them for the Isotropic phase function */
+    rec->normal = -normalize(ray.direction);
+    rec->front_face = true;
+    rec->u = r1.u;
+    rec->v = r1.v;
+    rec->material = phase;
+    rec->prim_id = id;
+    return true;
+  }
+  const AABB *aabb() const override { return boundary->aabb(); }
+  Float pdf_value(V3 o, V3 d, int wl, Float t) const override { return boundary->pdf_value(o, d, wl, t); }
+  V3 centroid() const override { return boundary->centroid(); }
+  Hitable *clone() const override { return new ConstantMedium(HPtr(boundary->clone()), -1.0f / neg_inv_density, phase, id); }
+};
+
 struct BVHNode : Hitable {    /* bvh.rs:21-28, bvh/hitable_trait.rs */
   HPtr left, right; AABB box;
   BVHNode(HPtr l, HPtr r, AABB b) : left(std::move(l)), right(std::move(r)), box(b) {}
@@ -1031,6 +1089,14 @@ static HPtr make_object(const clv_scene_desc *d, uint32_t i, const std::vector<M
     case CLV_OBJ_TRIANGLE_QUV: h.reset(new Planar(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]), m, pid, true)); break;
     case CLV_OBJ_BOXY: h.reset(new Boxy(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), m, pid)); break;
     case CLV_OBJ_MOVING_SPHERE: h.reset(new MovingSphere(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), p[6], m, pid)); break;
+    case CLV_OBJ_CONSTANT_MEDIUM_SPHERE:  /* objects.rs:99-103; the boundary's material is never read */
+      if (m->kind != CLV_MAT_ISOTROPIC) { *err = "a medium's material must be Isotropic"; return HPtr(); }
+      h.reset(new ConstantMedium(HPtr(new Sphere(v3(p[0], p[1], p[2]), p[3], m, pid)), p[4], m, pid));
+      break;
+    case CLV_OBJ_CONSTANT_MEDIUM_BOXY:
+      if (m->kind != CLV_MAT_ISOTROPIC) { *err = "a medium's material must be Isotropic"; return HPtr(); }
+      h.reset(new ConstantMedium(HPtr(new Boxy(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), m, pid)), p[6], m, pid));
+      break;
     default: *err = "unknown object kind"; return HPtr();
   }
   if (o.xform != CLV_NO_XFORM) {
@@ -1070,7 +1136,7 @@ static Scene *scene_create(const clv_scene_desc *d, int bvh_alg, std::string *er
         m.kind == CLV_MAT_DIELECTRIC)
       m.spd = make_spd(cm.color);
     if (m.tex == CLV_TEX_SPATIAL_CHECKER || m.tex == CLV_TEX_SURFACE_CHECKER) m.spd_odd = make_spd(cm.odd_color);
-    if (m.kind > CLV_MAT_CONE_LIGHT || m.tex > CLV_TEX_SURFACE_CHECKER) { *err = "unknown material or texture kind"; return nullptr; }
+    if (m.kind > CLV_MAT_ISOTROPIC || m.tex > CLV_TEX_SURFACE_CHECKER) { *err = "unknown material or texture kind"; return nullptr; }
     if (m.tex == CLV_TEX_ILLUMINANT_D65) { m.illum = cie_d65; m.illum_first = CIE_D65_FIRST_NM; m.illum_count = CIE_D65_COUNT; }
     if (m.tex == CLV_TEX_ILLUMINANT_D50) { m.illum = cie_d50; m.illum_first = CIE_D50_FIRST_NM; m.illum_count = CIE_D50_COUNT; }
     s->materials.push_back(m);
@@ -1102,6 +1168,7 @@ static void trace(const Ray &ray, const Scene &scene, uint32_t depth, uint32_t m
   for (int i = 0; i < 4; i++) bg[i] = scene.background[wls[i] - MIN_WAVELENGTH];
   if (depth > max_depth) { for (int i = 0; i < 4; i++) out[i] = 0.0f; return; }
   HitRecord rec;
+  medium_key = rng->state;
   if (!scene.bvh_root->hit(ray, EPSILON_SHADOW_ACNE, FLT_MAX, &rec)) { for (int i = 0; i < 4; i++) out[i] = bg[i]; return; }
   const Material *m = rec.material;
   Float emitted[4];
@@ -1121,19 +1188,26 @@ static void trace(const Ray &ray, const Scene &scene, uint32_t depth, uint32_t m
     for (int i = 0; i < 4; i++) out[i] = spec[i] * att[i];
     return;
   }
-  /* Diffuse: MixturePDF(HitablePDF(mis root, position), CosinePDF) -- pdf.rs:106-141 */
+  /* Diffuse: MixturePDF(HitablePDF(mis root, position), CosinePDF or SpherePDF) -- pdf.rs:106-168 */
   V3 gen;
   if (orc_bool(rng)) {
     scene.mis_root->random(rec.position, rng, &gen);
+  } else if (sr.sphere) {
+    gen = random_unit_vector(rng);
   } else {
     gen = sr.cosine_uvw.local(random_cosine_direction(rng));
   }
   V3 direction = normalize(gen);
   Ray sray; sray.origin = rec.position; sray.direction = direction; sray.time = ray.time; sray.wavelength = ray.wavelength;
   Float light_pdf = scene.mis_root->pdf_value(rec.position, direction, hero, ray.time);
-  Float cosine = dot(normalize(direction), sr.cosine_uvw.w);
-  Float cos_pdf = cosine <= 0.0f ? 0.0f : cosine / PI;
-  Float mis_pdf = 0.5f * light_pdf + 0.5f * cos_pdf;
+  Float mat_pdf;
+  if (sr.sphere) {
+    mat_pdf = 1.0f / (4.0f * PI);
+  } else {
+    Float cosine = dot(normalize(direction), sr.cosine_uvw.w);
+    mat_pdf = cosine <= 0.0f ? 0.0f : cosine / PI;
+  }
+  Float mis_pdf = 0.5f * light_pdf + 0.5f * mat_pdf;
   Float spdf;
   if (!m->scattering_pdf(rec, sray, &spdf)) { for (int i = 0; i < 4; i++) out[i] = emitted[i]; return; }
   Float rec4[4];
@@ -1277,6 +1351,7 @@ int orc_primary_hits(const orc_scene *o, const clv_render_opts *opts, uint32_t *
       orc_pcg rng = orc_pcg_for_path(opts->seed, x + y * opts->width, opts->sample_offset);
       Ray ray = camera_sample(s, cam, x, y, opts->width, opts->height, &rng);
       HitRecord rec;
+      medium_key = rng.state;
       size_t oi = (size_t)r * opts->width + x;
       if (s.bvh_root->hit(ray, EPSILON_SHADOW_ACNE, FLT_MAX, &rec)) {
         ids[oi] = rec.prim_id;
@@ -1306,6 +1381,7 @@ int orc_normal_map(const orc_scene *o, const clv_render_opts *opts, float *rgb) 
       Float time = orc_uniform(&rng);
       Ray ray = get_ray(cam, (Float)x / (Float)opts->width, (Float)y / (Float)opts->height, 0.0f, 0.0f, time, wl);
       HitRecord rec;
+      medium_key = rng.state;
       float *px = rgb + 3 * ((size_t)r * opts->width + x);
       if (!s.bvh_root->hit(ray, EPSILON_SHADOW_ACNE, FLT_MAX, &rec)) {
         px[0] = px[1] = px[2] = 0.0f;

@@ -279,3 +279,63 @@ def motion_check(img, sq, spp, W, H):
     se_tot = np.sqrt(np.maximum(sq - img ** 2, 0).sum((0, 1)) / spp)
     assert np.all(np.abs(tot - want.sum((0, 1))) < 4 * se_tot), (tot, want.sum((0, 1)), se_tot)
     return cov
+
+
+# --------------------------------------------------------------------------
+# Constant media (objects/constant_medium.rs).  The probe camera looks down -z
+# from z = 10 with a 0.05-degree field of view, so every primary ray runs
+# within 5e-3 of the axis; camera rays are unit length (camera.rs:115-119), so
+# a ray's parameter is its world distance.  A lamp sphere far out of view is
+# the light-sampling target the reference requires.
+MEDIUM_CAM = {"look_from": [0, 0, 10], "look_at": [0, 0, 0], "up": [0, 1, 0], "vertical_fov": 0.05,
+              "aperture": 0, "focus_distance": 1}
+
+
+def medium_probe(boundary, density, outer=None):
+    """One medium (object 0, report id 0) with the given boundary JSON,
+    optionally wrapped by outer(obj)."""
+    med = {"kind": "ConstantMedium", "density": density, "boundary": boundary,
+           "texture": {"kind": "SolidColor", "color": {"xyz_e": [0.5, 0.5, 0.5]}}}
+    if outer:
+        med = outer(med)
+    return load_scene({
+        "time_0": 0, "time_1": 1, "camera": MEDIUM_CAM, "background_color": [0, 0, 0],
+        "objects": [med, {"kind": "Sphere", "priority": True, "center": [0, 50, 0], "radius": 1, "material": "lamp"}],
+        "materials": [{"name": "lamp", "kind": "DiffuseLight", "emit": {"kind": "SolidColor", "color": {"xyz_e": [1, 1, 1]}}}]})
+
+
+def free_flight_check(ids, ts, entry, chord, sigma, label):
+    """Primary hits through a medium of optical depth sigma * chord: the medium
+    (id 0) is hit with probability 1 - exp(-sigma chord), and the hit lies an
+    exponential distance of rate sigma past the entry, truncated at the chord."""
+    from scipy import stats
+    n = ids.size
+    hit = ids == 0
+    p = 1.0 - np.exp(-sigma * chord)
+    se = np.sqrt(p * (1 - p) / n)
+    frac = hit.mean()
+    print("%s: hit fraction %.4f want %.4f (se %.4f)" % (label, frac, p, se))
+    assert abs(frac - p) < 4 * se + 1e-6, (label, frac, p)
+    assert np.all((ids == 0) | (ids == 0xFFFFFFFF)), label
+    x = ts[hit].astype(np.float64) - entry
+    assert x.min() > -1e-3 and x.max() < chord + 1e-3, (label, x.min(), x.max())
+    norm = 1.0 - np.exp(-sigma * chord)
+    ks = stats.kstest(np.clip(x, 0, chord), lambda v: (1.0 - np.exp(-sigma * np.clip(v, 0, chord))) / norm)
+    print("%s: KS p = %.3g over %d hits" % (label, ks.pvalue, hit.sum()))
+    assert ks.pvalue > 1e-3, (label, ks)
+
+
+def medium_furnace(density=1.0, background=0.5):
+    """A unit fog sphere of white Isotropic albedo (Xyz<E>(1,1,1): a flat
+    spectrum) under a uniform grey sky: nothing absorbs, every path ends in
+    the sky, so every pixel's expectation is the sky's (with a sky-radiance
+    lamp sphere as the light-sampling target, as in furnace())."""
+    from clovers_amd.color import srgb_eotf
+    g = float(srgb_eotf(background))
+    return load_scene({
+        "time_0": 0, "time_1": 1, "camera": CAM, "background_color": [background] * 3,
+        "objects": [{"kind": "ConstantMedium", "density": density,
+                     "texture": {"kind": "SolidColor", "color": {"xyz_e": [1, 1, 1]}},
+                     "boundary": {"kind": "Sphere", "center": [0, 0, 0], "radius": 1.0}},
+                    {"kind": "Sphere", "priority": True, "center": [0, 3, 0], "radius": 0.5, "material": "sky"}],
+        "materials": [{"name": "sky", "kind": "DiffuseLight", "emit": {"kind": "SolidColor", "color": {"xyz_e": [g] * 3}}}]})

@@ -36,7 +36,8 @@ def test_arguments(args, code):
 
 def test_scene_errors_are_reported(tmp_path):
     sf = synth.cornell_json()
-    sf["objects"].append({"kind": "ConstantMedium", "boundary": {"kind": "Sphere", "center": [0, 0, 0], "radius": 1}})
+    sf["objects"].append({"kind": "ConstantMedium", "boundary": {"kind": "Quad", "q": [0, 0, 0], "u": [1, 0, 0],
+                                                                 "v": [0, 1, 0]}})
     p = tmp_path / "s.json"
     p.write_text(json.dumps(sf))
     r = run("-i", p)

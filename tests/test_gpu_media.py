@@ -1,0 +1,130 @@
+"""Constant media and the Isotropic phase function on the device: the closed
+forms the oracle is pinned by (tests/test_oracle_media.py), hit-for-hit
+parity with the oracle (the free-flight uniform is keyed by the path's stream
+state and the medium, so both draw the same one), statistical parity of full
+renders, and the batch / builder plumbing around the media pass."""
+import numpy as np
+import pytest
+
+from clovers_amd import abi, synth
+from oracle import py_oracle as po
+import special_scenes as S
+from test_gpu_parity import block_z
+
+pytestmark = pytest.mark.gpu
+
+SPHERE = {"kind": "Sphere", "center": [0, 0, 0], "radius": 1.0}
+BOX45 = {"kind": "RotateY", "angle": 45, "object": {"kind": "Boxy", "corner_0": [-1, -1, -1], "corner_1": [1, 1, 1]}}
+
+
+def hits(r, scene, W=128, seed=3, builder="host"):
+    sid = r.upload(scene, builder=builder)
+    try:
+        return r.primary_hits(sid, abi.render_opts(W, W, 1, 1, seed=seed))
+    finally:
+        r.free(sid)
+
+
+@pytest.mark.parametrize("boundary,sigma,entry,chord", [
+    (SPHERE, 0.5, 9.0, 2.0),
+    (BOX45, 0.4, 10.0 - np.sqrt(2.0), 2.0 * np.sqrt(2.0)),
+    ({"kind": "Sphere", "center": [0, 0, 0], "radius": 100.0}, 0.2, 0.001, 110.0 - 0.001),
+], ids=["sphere", "box-diagonal", "inside"])
+def test_free_flight_closed_form(renderer, boundary, sigma, entry, chord):
+    ids, ts = hits(renderer, S.medium_probe(boundary, sigma))
+    S.free_flight_check(ids, ts, entry, chord, sigma, "gpu")
+
+
+@pytest.mark.parametrize("boundary", [SPHERE, BOX45], ids=["sphere", "box"])
+def test_primary_hits_match_oracle(renderer, boundary):
+    """same stream state, same medium: the same free-flight uniform, so the
+    same hits and (to the last bits of logf) the same distances"""
+    s = S.medium_probe(boundary, 0.6)
+    gi, gt = hits(renderer, s)
+    oi, ot = po.OracleScene(s).primary_hits(abi.render_opts(128, 128, 1, 1, seed=3))
+    same = gi == oi
+    print("%.5f ids equal" % same.mean())
+    assert same.mean() >= 0.999
+    m = same & (gi == 0)
+    assert m.sum() > 1000
+    assert np.allclose(gt[m], ot[m], rtol=2e-6, atol=0)
+
+
+def test_media_room_primary_hits(renderer):
+    s = synth.media()
+    o = abi.render_opts(160, 120, 1, 8, seed=4)
+    sid = renderer.upload(s)
+    gi, gt = renderer.primary_hits(sid, o)
+    renderer.free(sid)
+    oi, ot = po.OracleScene(s).primary_hits(o)
+    same = gi == oi
+    print("%.5f ids equal" % same.mean())
+    assert same.mean() >= 0.999
+    kinds = s.objects["kind"]
+    media = np.nonzero((kinds == abi.OBJ_MEDIUM_SPHERE) | (kinds == abi.OBJ_MEDIUM_BOXY))[0] * 8
+    assert np.isin(gi, media).mean() > 0.2  # the camera sees the fog
+
+
+def test_white_medium_furnace(renderer):
+    s = S.medium_furnace(1.0)
+    W = H = 48
+    sid = renderer.upload(s)
+    img, sq = renderer.render(sid, abi.render_opts(W, H, 256, 64, seed=5), want_sq=True)
+    renderer.free(sid)
+    img = img.astype(np.float64)
+    g = float(s.background[0])
+    want = S.spd_xyz((g, g, g))
+    mask = S.sphere_mask(W, H)
+    for label, m in (("fog", mask), ("sky", ~mask)):
+        mean = img[m].mean(0)
+        var = (sq[m] - img[m] ** 2).mean(0)
+        se = np.sqrt(np.maximum(var, 0) / (m.sum() * 256)) + 1e-9
+        print(label, mean, want, se)
+        assert np.all(np.abs(mean - want) < 4 * se + 2e-5 * want), (label, mean, want, se)
+
+
+def test_media_room_matches_oracle_statistically(renderer):
+    s = synth.media()
+    W, H, spp = 96, 72, 64
+    sid = renderer.upload(s)
+    g, gsq = renderer.render(sid, abi.render_opts(W, H, spp, 12, seed=1001), want_sq=True)
+    renderer.free(sid)
+    c, csq, _ = po.OracleScene(s).render(abi.render_opts(W, H, spp, 12, seed=2002), want_sq=True)
+    z, _, _ = block_z(g, gsq, c, csq, spp, spp, 8)
+    print("%d block-channels, mean z^2 %.3f, max |z| %.2f" % (z.size, np.mean(z * z), np.abs(z).max()))
+    assert np.mean(np.abs(z) > 3) <= 0.02
+    assert np.abs(z).max() < 6
+
+
+def test_media_room_same_streams_same_paths(renderer):
+    s = synth.media()
+    o = abi.render_opts(96, 72, 1, 12, seed=7)
+    sid = renderer.upload(s)
+    g, _ = renderer.render(sid, o)
+    renderer.free(sid)
+    c, _, _ = po.OracleScene(s).render(o)
+    close = np.all(np.abs(g - c) <= 1e-3 * np.abs(c) + 1e-6, axis=-1)
+    print("%.4f of pixels equal to float rounding" % close.mean())
+    assert close.mean() > 0.95
+
+
+def test_media_scene_in_a_batch_and_with_the_device_builder(renderer):
+    """a batch with a media scene traces every job with the media-capable
+    kernel: a plain scene's job comes out as when rendered alone; the device
+    builder places the media records after its tree's"""
+    m_host = renderer.upload(synth.media())
+    m_dev = renderer.upload(synth.media(), builder="device")
+    c = renderer.upload(synth.cornell())
+    try:
+        om = abi.render_opts(64, 48, 8, 12, seed=2)
+        oc = abi.render_opts(64, 64, 8, 8, seed=4)
+        outs = renderer.render_batch([(m_host, om), (c, oc)])
+        ref_c, _ = renderer.render(c, oc)
+        ref_m, _ = renderer.render(m_host, om)
+        assert np.array_equal(outs[1][0], ref_c)
+        assert np.array_equal(outs[0][0], ref_m)
+        dev_m, _ = renderer.render(m_dev, om)
+        assert np.array_equal(dev_m, ref_m)
+    finally:
+        for sid in (c, m_dev, m_host):
+            renderer.free(sid)

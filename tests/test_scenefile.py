@@ -92,8 +92,19 @@ def test_object_list_flattening_and_priority():
 def test_unsupported_inputs_raise():
     sf = synth.cornell_json()
     sf["objects"].append({"kind": "ConstantMedium", "density": 0.1,
+                          "boundary": {"kind": "Quad", "q": [0, 0, 0], "u": [1, 0, 0], "v": [0, 1, 0]}})
+    with pytest.raises(NotImplementedError):
+        load_scene(sf)
+    sf = synth.cornell_json()
+    sf["objects"].append({"kind": "ConstantMedium", "density": 0.1,
+                          "texture": {"kind": "SurfaceChecker"},
                           "boundary": {"kind": "Sphere", "center": [0, 0, 0], "radius": 1}})
     with pytest.raises(NotImplementedError):
+        load_scene(sf)
+    sf = synth.cornell_json()
+    sf["objects"].append({"kind": "ConstantMedium", "priority": True,
+                          "boundary": {"kind": "Sphere", "center": [0, 0, 0], "radius": 1}})
+    with pytest.raises(ValueError):
         load_scene(sf)
     sf = synth.cornell_json()
     sf["objects"].append({"kind": "Translate", "offset": [0, 0, 0],

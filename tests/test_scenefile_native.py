@@ -37,10 +37,10 @@ def same(py, nat):
         assert np.allclose(v, nat.camera[k]), k
 
 
-@pytest.mark.parametrize("name", ["cornell", "spheres", "moving", "prism", "features"])
+@pytest.mark.parametrize("name", ["cornell", "spheres", "moving", "prism", "features", "media"])
 def test_synthetic_scenes_agree(name):
     js = {"cornell": synth.cornell_json, "spheres": synth.spheres_json, "moving": lambda: synth.spheres_json(3, True),
-          "prism": synth.prism_json, "features": synth.features_json}[name]
+          "prism": synth.prism_json, "features": synth.features_json, "media": synth.media_json}[name]
     js = js()
     nat = clovers_amd.load_scene_file(json.dumps(js))
     same(load_scene(js), nat)
@@ -83,13 +83,17 @@ def test_meshes_and_colours_agree(tmp_path):
 
 
 @pytest.mark.parametrize("mutate,status", [
-    (lambda s: s["objects"].append({"kind": "ConstantMedium", "boundary": {"kind": "Sphere", "center": [0, 0, 0],
-                                                                             "radius": 1}}), abi.EUNSUPPORTED),
+    (lambda s: s["objects"].append({"kind": "ConstantMedium", "boundary": {"kind": "Quad", "q": [0, 0, 0],
+                                                                             "u": [1, 0, 0], "v": [0, 1, 0]}}),
+     abi.EUNSUPPORTED),
+    (lambda s: s["objects"].append({"kind": "ConstantMedium", "priority": True,
+                                    "boundary": {"kind": "Sphere", "center": [0, 0, 0], "radius": 1}}), abi.EINVAL),
     (lambda s: s["objects"].append({"kind": "Translate", "offset": [0, 0, 0],
                                     "object": {"kind": "ObjectList", "objects": []}}), abi.EUNSUPPORTED),
     (lambda s: s["objects"].append({"kind": "ObjectList", "objects": [{"kind": "ObjectList", "objects": []}]}),
      abi.EINVAL),
-    (lambda s: s["materials"].append({"name": "x", "kind": "Isotropic"}), abi.EUNSUPPORTED),
+    (lambda s: s["materials"].append({"name": "x", "kind": "Isotropic", "albedo": {"kind": "SurfaceChecker"}}),
+     abi.EUNSUPPORTED),
     (lambda s: s["materials"].append({"name": "x", "kind": "ConeLight", "spread": 3}), abi.EINVAL),
     (lambda s: s.pop("camera"), abi.EINVAL),
     (lambda s: s["objects"].append({"kind": "STL", "path": "/nonexistent.stl", "scale": 1, "center": [0, 0, 0],
