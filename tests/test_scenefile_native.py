@@ -126,7 +126,8 @@ def test_json_strings_and_numbers():
 
 
 @has_ref
-@pytest.mark.parametrize("name", ["cornell", "colorchecker", "dispersive", "one_weekend", "boing", "teapot", "bunny"])
+@pytest.mark.parametrize("name", ["cornell", "colorchecker", "dispersive", "one_weekend", "boing", "teapot", "bunny",
+                                  "spatial_checker_smoke"])
 def test_reference_scenes_agree(name):
     path = os.path.join(REF, "scenes", name + ".json")
     nat = clovers_amd.load_scene_file(path, mesh_root=REF)
