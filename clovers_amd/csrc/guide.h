@@ -173,6 +173,21 @@ __device__ __forceinline__ void gd_forward_tile(const _Float16 *__restrict__ Wh,
 /* the forward pass with weights (W1..W4 stacked row-major, rows padded to
  * GD_WLD halfs) and biases (b1 b2 b3 b4) staged in LDS */
 #define GD_WLD 72
+/* The 64-deep contractions run as two 16x16x32 MFMAs.  Within a 32-deep step
+ * s, the k positions a lane group supplies are taken as rows
+ * {32 s + 4 g + i} and {32 s + 16 + 4 g + i} (i < 4) -- exactly what the lane
+ * holds of the previous layer's tiles 2s and 2s+1 -- and the weights are read
+ * in the same order, so the activations still never leave registers. */
+typedef _Float16 gh8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ gh8 gd_cat(gh4 lo, gh4 hi) {
+  gh8 v;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    v[i] = lo[i];
+    v[4 + i] = hi[i];
+  }
+  return v;
+}
 __device__ __forceinline__ void gd_forward_tile_lds(const _Float16 *W, const float *b, const gh4 x[4], gf4 out[3]) {
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   gh4 cur[4];
@@ -188,10 +203,11 @@ __device__ __forceinline__ void gd_forward_tile_lds(const _Float16 *W, const flo
       gf4 c;
 #pragma unroll
       for (int i = 0; i < 4; i++) c[i] = b[GD_HID * l + 16 * mt + 4 * g + i];
+      const _Float16 *wrow = W + (GD_HID * l + 16 * mt + r) * GD_WLD + 4 * g;
 #pragma unroll
-      for (int kt = 0; kt < 4; kt++) {
-        const gh4 a = *reinterpret_cast<const gh4 *>(W + (GD_HID * l + 16 * mt + r) * GD_WLD + 16 * kt + 4 * g);
-        c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, cur[kt], c, 0, 0, 0);
+      for (int st = 0; st < 2; st++) {
+        const gh8 a = gd_cat(*reinterpret_cast<const gh4 *>(wrow + 32 * st), *reinterpret_cast<const gh4 *>(wrow + 32 * st + 16));
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, gd_cat(cur[2 * st], cur[2 * st + 1]), c, 0, 0, 0);
       }
       acc[mt] = c;
     }

@@ -404,12 +404,25 @@ __global__ __launch_bounds__(256) void k_guide_halves(const float *P, _Float16 *
 }
 
 /* network probe: raw outputs (40) and decoded lobes (40) of n inputs
- * (p xyz, n xyz, lambda; 7 floats each) */
+ * (p xyz, n xyz, lambda; 7 floats each), through the rendering path's forward
+ * pass (weights staged in LDS, 16x16x32 MFMAs) */
 __global__ __launch_bounds__(64) void k_guide_eval(const float *in, uint32_t n, const _Float16 *Wh, const float *P,
                                                    float *raw, float *lobes) {
+  __shared__ __attribute__((aligned(16))) _Float16 wl[(3 * GD_HID + GD_OUT_PAD) * GD_WLD];
+  __shared__ float bl[GD_HID * 3 + GD_OUT_PAD];
   __shared__ float rin[64][8];
   __shared__ float OG[64 * GT_OG];
   const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+  {
+    const uint4 *src = reinterpret_cast<const uint4 *>(Wh);
+    for (uint32_t k = threadIdx.x; k < GD_H_WT(0) / 8u; k += 64u)
+      *reinterpret_cast<uint4 *>(wl + (k >> 3) * GD_WLD + (k & 7u) * 8u) = src[k];
+    for (uint32_t k = threadIdx.x; k < GD_HID * 3u + GD_OUT_PAD; k += 64u) {
+      const uint32_t l = k / GD_HID;
+      bl[k] = P[l == 0 ? GD_B1 + k : (l == 1 ? GD_B2 + k - GD_HID : (l == 2 ? GD_B3 + k - 2 * GD_HID
+                                                                               : GD_B4 + k - 3 * GD_HID))];
+    }
+  }
   const uint32_t i = blockIdx.x * 64u + (uint32_t)lane;
   for (int k = 0; k < 7; k++) rin[lane][k] = i < n ? in[7 * (size_t)i + k] : 0.0f;
   __syncthreads();
@@ -419,7 +432,7 @@ __global__ __launch_bounds__(64) void k_guide_eval(const float *in, uint32_t n, 
     gh4 x[4];
     gf4 out[3];
     gd_encode(p, nn, rin[q][6], x);
-    gd_forward_tile(Wh, P, x, out, nullptr);
+    gd_forward_tile_lds(wl, bl, x, out);
 #pragma unroll
     for (int mt = 0; mt < 3; mt++)
 #pragma unroll

@@ -1349,36 +1349,47 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathBuf pb, ShadeQue
 }
 
 /* Learned-proposal lobes for every guided path of the Lambertian queue.
- * Resident blocks of 4 waves stage the fp16 weights and the biases in LDS
- * once, then loop over the queue 256 paths at a time: one lane per path
+ * Resident blocks of 8 waves stage the fp16 weights and the biases in LDS
+ * once (one copy serves 8 waves, so 4 waves per SIMD fit), then every wave
+ * loops over the queue on its own, 64 paths at a time: one lane per path
  * reconstructs the hit (position normalised to the job's scene box, oriented
- * normal, hero wavelength); each wave runs the MLP on MFMA for its 64 paths,
+ * normal, hero wavelength); the wave runs the MLP on MFMA for its 64 paths,
  * 16 at a time, and 4 lanes per path decode 2 lobes each straight to the
- * lobe buffer.  The loop is block-uniform, so LDS exchanges use __syncthreads. */
+ * lobe buffer.  A wave exchanges data only through its own LDS slices, so
+ * the loop has no block barrier: a wave's LDS accesses complete in order, and
+ * gi_wave_sync keeps the compiler from moving them across an exchange. */
 #define GI_OG 52 /* LDS row stride (floats) of one tile's outputs */
-__global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, GuideArgs ga, const uint32_t *queue,
-                                                     const uint32_t *count) {
+#define GI_WAVES 8
+#define GI_BLOCK (64 * GI_WAVES)
+__device__ __forceinline__ void gi_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__global__ __launch_bounds__(GI_BLOCK) void k_guide_infer(DevScene sc, PathBuf pb, GuideArgs ga, const uint32_t *queue,
+                                                          const uint32_t *count) {
   /* W1..W4 row-major with rows padded to GD_WLD halfs (bank-conflict-free A operands) */
   __shared__ __attribute__((aligned(16))) _Float16 wl[(3 * GD_HID + GD_OUT_PAD) * GD_WLD];
   __shared__ float bl[GD_HID * 3 + GD_OUT_PAD];
-  __shared__ float rin[4][64][8];
-  __shared__ float og[4][16 * GI_OG];
+  __shared__ float rin[GI_WAVES][64][8];
+  __shared__ float og[GI_WAVES][16 * GI_OG];
   {
     /* 8 uint4 per 64-half row; row i of the stacked W1..W4 goes to LDS row i */
     const uint4 *src = reinterpret_cast<const uint4 *>(ga.Wh);
-    for (uint32_t k = threadIdx.x; k < GD_H_WT(0) / 8u; k += 256u)
+    for (uint32_t k = threadIdx.x; k < GD_H_WT(0) / 8u; k += GI_BLOCK)
       *reinterpret_cast<uint4 *>(wl + (k >> 3) * GD_WLD + (k & 7u) * 8u) = src[k];
-    for (uint32_t k = threadIdx.x; k < GD_HID * 3u + GD_OUT_PAD; k += 256u) {
+    for (uint32_t k = threadIdx.x; k < GD_HID * 3u + GD_OUT_PAD; k += GI_BLOCK) {
       const uint32_t l = k / GD_HID;
       bl[k] = ga.P[l == 0 ? GD_B1 + k : (l == 1 ? GD_B2 + k - GD_HID : (l == 2 ? GD_B3 + k - 2 * GD_HID
                                                                                  : GD_B4 + k - 3 * GD_HID))];
     }
   }
+  __syncthreads(); /* the weights are shared by the block's waves */
   const uint32_t n = *count;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   if (blockIdx.x == 0 && threadIdx.x == 0 && ga.stats) atomicAdd(ga.stats + 3, (unsigned long long)n);
-  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
-    const uint32_t qi = base + threadIdx.x;
+  for (uint32_t base = blockIdx.x * GI_BLOCK + 64u * (uint32_t)w; base < n; base += gridDim.x * GI_BLOCK) {
+    const uint32_t qi = base + lane;
     uint32_t path = 0xFFFFFFFFu;
     float in[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (qi < n) {
@@ -1401,7 +1412,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, Gu
 #pragma unroll
     for (int k = 0; k < 8; k++) rin[w][lane][k] = in[k];
     rin[w][lane][7] = __uint_as_float(path);
-    __syncthreads();
+    gi_wave_sync();
 #pragma unroll 1
     for (int nt = 0; nt < 4; nt++) {
       const int q = 16 * nt + r;
@@ -1417,7 +1428,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, Gu
 #pragma unroll
           for (int k = 0; k < 4; k++) og[w][r * GI_OG + 16 * mt + 4 * g + k] = out[mt][k];
       }
-      __syncthreads();
+      gi_wave_sync();
       const uint32_t pth = __float_as_uint(rin[w][q][7]);
       if (pth != 0xFFFFFFFFu) {
         /* lane g decodes lobes 2g, 2g+1 of path pth; the softmax needs all 8 logits */
@@ -1449,7 +1460,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(DevScene sc, PathBuf pb, Gu
         dst[1] = make_float4(v[4], v[5], v[6], v[7]);
         dst[2] = make_float4(v[8], v[9], v[10], v[11]);
       }
-      __syncthreads();
+      gi_wave_sync();
     }
   }
 }
@@ -1731,11 +1742,11 @@ void launch_guide_infer(hipStream_t st, const DevScene &sc, const PathBuf &pb, c
     int per_cu = 0, dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_guide_infer, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_guide_infer, GI_BLOCK, 0);
     resident = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
   }
-  hipLaunchKernelGGL(k_guide_infer, dim3(std::min(resident, grid_for(max_paths, 256, 1u << 30))), dim3(256), 0, st,
-                     sc, pb, ga, queue, count);
+  hipLaunchKernelGGL(k_guide_infer, dim3(std::min(resident, grid_for(max_paths, GI_BLOCK, 1u << 30))), dim3(GI_BLOCK), 0,
+                     st, sc, pb, ga, queue, count);
 }
 
 void launch_resolve(hipStream_t st, const PathBuf &pb, const ResolveParams &rp, const float4 *cie) {
