@@ -163,7 +163,7 @@ def run_prism(args):
         pts = []
         for spp in spps:
             mse, ms = 0.0, 0.0
-            seeds = (7, 8)
+            seeds = tuple(range(7, 7 + args.seeds))
             for sd in seeds:
                 img, _ = r.render(sid, abi.render_opts(W, H, spp, D, seed=sd, guiding=g))
                 ms += r.counters()["render_ms"] / len(seeds)
@@ -181,13 +181,14 @@ def run_prism(args):
     r.close()
     line = {
         "metric": "time-to-RMSE vs oracle, guided vs plain", "value": round(ratio, 3),
-        "unit": "x (plain device time / guided device time at equal RMSE)", "n_gpus": 1, "steps": 2 * len(spps),
+        "unit": "x (plain device time / guided device time at equal RMSE)", "n_gpus": 1,
+        "steps": 2 * len(spps) * args.seeds,
         "warmup": 0, "ms_per_step": round(t_guided, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (network fp16 on MFMA, fp32 accumulate)", "data": "synthetic",
         "config": {"workload": "prism %dx%d depth%d" % (W, H, D), "reference_spp": args.ref_spp,
                    "reference": "plain method, float64 accumulation", "target_rmse": target,
                    "plain_ms_at_target": t_plain, "guided_ms_at_target": round(t_guided, 3),
-                   "guided_extrapolated": extrapolated, "curves_ms_rmse": curves, "seeds_per_point": 2,
+                   "guided_extrapolated": extrapolated, "curves_ms_rmse": curves, "seeds_per_point": args.seeds,
                    "tuning": dict(tuning, trace_kernel=abi.TRACE_NAMES[tuning["trace_kernel"]])},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_FP16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / MFMA_FP16_PEAK_TFLOPS, 5), "traffic": None,
@@ -215,6 +216,9 @@ def main():
     ap.add_argument("--trace-kernel", default="", choices=["", "simple", "persistent", "ww", "lds_cached"],
                     help="fix the traversal kernel instead of autotuning")
     ap.add_argument("--ref-spp", type=int, default=65536, help="prism workload: samples of the reference image")
+    ap.add_argument("--seeds", type=int, default=4,
+                    help="prism workload: independent renders per curve point (the error is dominated by rare "
+                         "caustic paths, so one render's RMSE is noisy)")
     ap.add_argument("--guide-alpha", type=float, default=0.0, help="prism workload: override the guide's share")
     ap.add_argument("--guide-rounds", type=int, default=0, help="prism workload: override the training rounds")
     ap.add_argument("--guide-train-spp", type=int, default=0,

@@ -92,7 +92,7 @@ struct GuideState {
   float lr = 0.005f;
   uint32_t records = 2097152;
   float wclip = 1e6f;
-  uint32_t rounds = 4;
+  uint32_t rounds = 2;
   uint32_t credit = 1;       /* clv_tuning.guide_credit */
   uint32_t step = GUIDE_STEP; /* records per Adam step */
 };

@@ -350,7 +350,7 @@ typedef struct {
   float guide_lr;           /* Adam learning rate (default 0.005) */
   uint32_t guide_records;   /* training records used per round (default 2^21, in steps of 16384) */
   float guide_wclip;        /* training weight F/p capped at this multiple of its mean (default 1e6) */
-  uint32_t guide_rounds;    /* training rounds per guided render (default 4); later samples use the
+  uint32_t guide_rounds;    /* training rounds per guided render (default 2); later samples use the
                                trained network in large passes without training */
   uint32_t use_graph;       /* 1 (default): a render repeated with identical arguments is captured as a hipGraph
                                on its second call and replayed from then on; never for CLV_FLAG_TIME_KERNELS
