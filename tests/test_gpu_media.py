@@ -128,3 +128,36 @@ def test_media_scene_in_a_batch_and_with_the_device_builder(renderer):
     finally:
         for sid in (c, m_dev, m_host):
             renderer.free(sid)
+
+
+def test_guided_media_render_is_unbiased(renderer):
+    """guided shading never guides a medium vertex (the Isotropic phase
+    function keeps the plain light / sphere mixture): the guided image of the
+    media room agrees with the plain one within their errors"""
+    s = synth.media()
+    W, H, spp = 96, 72, 64
+    sid = renderer.upload(s)
+    try:
+        p, psq = renderer.render(sid, abi.render_opts(W, H, spp, 12, seed=11), want_sq=True)
+        g, gsq = renderer.render(sid, abi.render_opts(W, H, spp, 12, seed=12, guiding=abi.GUIDE_ON), want_sq=True)
+    finally:
+        renderer.free(sid)
+    assert np.isfinite(g).all()
+    z, _, _ = block_z(g, gsq, p, psq, spp, spp, 8)
+    print("%d block-channels, mean z^2 %.3f, max |z| %.2f" % (z.size, np.mean(z * z), np.abs(z).max()))
+    assert np.mean(np.abs(z) > 3) <= 0.02
+    assert np.abs(z).max() < 6
+
+
+def test_media_render_graph_replay(renderer):
+    """the second identical render is captured as a graph and later ones
+    replay it: every replay gives the eager bytes"""
+    sid = renderer.upload(synth.media())
+    try:
+        o = abi.render_opts(64, 48, 8, 12, seed=3)
+        first, _ = renderer.render(sid, o)
+        for _ in range(3):
+            again, _ = renderer.render(sid, o)
+            assert np.array_equal(again, first)
+    finally:
+        renderer.free(sid)
