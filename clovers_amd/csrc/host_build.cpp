@@ -370,52 +370,105 @@ struct BvhBuilder {
   }
 };
 
-struct Collapser {
+/* Cost-optimal collapse of the binary tree into 4-wide nodes (dynamic
+ * programming over the binary nodes, bottom-up): with A(n) the surface area
+ * of node n, the wide tree costs C_NODE * A per wide node plus C_PRIM * A *
+ * primitives per leaf.  D(n, k) is the least cost of the subtree of n as a
+ * forest of at most k parts -- F(n) itself, or D(left, j) + D(right, k - j) --
+ * where F(n) is the cheaper of a leaf (any subtree of at most LEAF_MAX_PRIMS
+ * primitives, whose records are contiguous in the build order) and a wide
+ * node, W(n) = C_NODE * A(n) + min_j D(left, j) + D(right, 4 - j).  Decisions
+ * are kept per node and read back top-down. */
+struct DpCollapser {
+  static constexpr float C_NODE = 1.0f, C_PRIM = 1.0f;
   const BvhBuilder &b;
   const std::vector<FlatPrim> &prims;
   const std::vector<float> &ext;
   std::vector<Bvh4Node> out;
   std::vector<float> records;
   uint32_t depth = 0;
-  Collapser(const BvhBuilder &bb, const std::vector<FlatPrim> &p, const std::vector<float> &e) : b(bb), prims(p), ext(e) {}
+  std::vector<float> D;          /* 4 per node: D(n, 1..4) */
+  std::vector<uint8_t> dsplit;   /* 4 per node: 0 = F(n) whole, else parts given to the left child */
+  std::vector<uint8_t> wsplit;   /* parts given to the left child by W(n) */
+  std::vector<uint8_t> leaf;     /* F(n) is a leaf */
+  std::vector<uint32_t> first, count;
+  DpCollapser(const BvhBuilder &bb, const std::vector<FlatPrim> &p, const std::vector<float> &e)
+      : b(bb), prims(p), ext(e) {}
 
-  uint32_t leaf_word(const BNode &n) {
-    uint32_t off = (uint32_t)(records.size() / 4);
-    for (uint32_t i = 0; i < n.count; i++) write_record(records, prims[b.perm[n.first + i]], ext);
-    return CHILD_LEAF | ((n.count - 1u) << 28) | off;
-  }
-  uint32_t collapse(int bi, uint32_t d) {
-    if (d > depth) depth = d;
-    uint32_t idx = (uint32_t)out.size();
-    out.emplace_back();
-    std::vector<int> ch;
-    const BNode &root = b.nodes[bi];
-    if (root.count > 0) {
-      ch.push_back(bi);
-    } else {
-      ch.push_back(root.left);
-      ch.push_back(root.right);
-    }
-    while (ch.size() < 4) {
-      int pick = -1;
-      float best = -1.0f;
-      for (size_t i = 0; i < ch.size(); i++) {
-        const BNode &c = b.nodes[ch[i]];
-        if (c.count == 0) {
-          float ar = box_area(c.box);
-          if (ar > best) { best = ar; pick = (int)i; }
-        }
+  void plan() {
+    const size_t n = b.nodes.size();
+    D.assign(4 * n, 0.0f);
+    dsplit.assign(4 * n, 0);
+    wsplit.assign(n, 0);
+    leaf.assign(n, 0);
+    first.assign(n, 0);
+    count.assign(n, 0);
+    for (size_t i = n; i-- > 0;) { /* children have larger indices than their parent */
+      const BNode &nd = b.nodes[i];
+      const float A = box_area(nd.box);
+      float F;
+      if (nd.count > 0) {
+        first[i] = nd.first;
+        count[i] = nd.count;
+        leaf[i] = 1;
+        F = C_PRIM * A * (float)nd.count;
+        for (int k = 0; k < 4; k++) D[4 * i + k] = F;
+        continue;
       }
-      if (pick < 0) break;
-      int n = ch[pick];
-      ch[pick] = b.nodes[n].left;
-      ch.insert(ch.begin() + pick + 1, b.nodes[n].right);
+      const size_t l = (size_t)nd.left, r = (size_t)nd.right;
+      first[i] = first[l];
+      count[i] = count[l] + count[r];
+      float best = FLT_MAX;
+      for (int j = 1; j <= 3; j++) {
+        const float c = D[4 * l + j - 1] + D[4 * r + 3 - j];
+        if (c < best) { best = c; wsplit[i] = (uint8_t)j; }
+      }
+      const float W = C_NODE * A + best;
+      const float L = count[i] <= LEAF_MAX_PRIMS ? C_PRIM * A * (float)count[i] : FLT_MAX;
+      leaf[i] = L <= W ? 1 : 0;
+      F = leaf[i] ? L : W;
+      D[4 * i] = F;
+      for (int k = 2; k <= 4; k++) {
+        float dk = F;
+        uint8_t sk = 0;
+        for (int j = 1; j < k; j++) {
+          const float c = D[4 * l + j - 1] + D[4 * r + k - j - 1];
+          if (c < dk) { dk = c; sk = (uint8_t)j; }
+        }
+        D[4 * i + k - 1] = dk;
+        dsplit[4 * i + k - 1] = sk;
+      }
+    }
+  }
+  void cut(int n, int k, std::vector<int> &parts) const {
+    const uint8_t j = k > 1 ? dsplit[4 * (size_t)n + k - 1] : 0;
+    if (j == 0) { parts.push_back(n); return; }
+    cut(b.nodes[n].left, j, parts);
+    cut(b.nodes[n].right, k - j, parts);
+  }
+  uint32_t leaf_word(int n) {
+    const uint32_t off = (uint32_t)(records.size() / 4);
+    for (uint32_t i = 0; i < count[n]; i++) write_record(records, prims[b.perm[first[n] + i]], ext);
+    return CHILD_LEAF | ((count[n] - 1u) << 28) | off;
+  }
+  /* the wide node of binary node n (a leaf root becomes a node with one leaf child) */
+  uint32_t emit(int n, uint32_t d) {
+    if (d > depth) depth = d;
+    const uint32_t idx = (uint32_t)out.size();
+    out.emplace_back();
+    std::vector<int> parts;
+    const BNode &nd = b.nodes[n];
+    if (nd.count > 0) {
+      parts.push_back(n);
+    } else {
+      cut(nd.left, wsplit[n], parts);
+      cut(nd.right, 4 - wsplit[n], parts);
     }
     Bvh4Node node;
     memset(&node, 0, sizeof(node));
     for (int k = 0; k < 4; k++) {
-      if (k < (int)ch.size()) {
-        const Box &bx = b.nodes[ch[k]].box;
+      if (k < (int)parts.size()) {
+        const Box &bx = b.nodes[parts[k]].box;
         node.lo_x[k] = bx.lo[0]; node.hi_x[k] = bx.hi[0];
         node.lo_y[k] = bx.lo[1]; node.hi_y[k] = bx.hi[1];
         node.lo_z[k] = bx.lo[2]; node.hi_z[k] = bx.hi[2];
@@ -425,9 +478,9 @@ struct Collapser {
         node.child[k] = CHILD_EMPTY;
       }
     }
-    for (int k = 0; k < (int)ch.size(); k++) {
-      const BNode &c = b.nodes[ch[k]];
-      node.child[k] = c.count > 0 ? leaf_word(c) : collapse(ch[k], d + 1);
+    for (int k = 0; k < (int)parts.size(); k++) {
+      const int c = parts[k];
+      node.child[k] = leaf[c] ? leaf_word(c) : emit(c, d + 1);
     }
     out[idx] = node;
     return idx;
@@ -1043,10 +1096,11 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
   } else {
     BvhBuilder bb(prims);
     int root = bb.build(0, (uint32_t)prims.size(), 0);
-    Collapser col(bb, prims, ext);
+    DpCollapser col(bb, prims, ext);
     col.out.reserve(bb.nodes.size() / 2 + 1);
     col.records.reserve(prims.size() * 16);
-    col.collapse(root, 0);
+    col.plan();
+    col.emit(root, 0);
     renumber_bfs(col.out);
     hs->nodes.swap(col.out);
     hs->prims.swap(col.records);
