@@ -410,23 +410,29 @@ __global__ __launch_bounds__(256) void k_cl_emit(uint32_t m, const uint4 *__rest
     lx[k] = lo.x; ly[k] = lo.y; lz[k] = lo.z;
     hx[k] = hi.x; hy[k] = hi.y; hz[k] = hi.z;
     if (is_leafy(id, bmeta)) {
+      /* the leaf's primitives (at most 4), copied in scene order: the sides of
+       * a Boxy that share a leaf are tested in the reference's side order */
       const uint32_t u0 = u_off;
-      uint32_t stk[8], sp = 0, cnt = 0;
+      uint32_t stk[8], sp = 0, cnt = 0, ps[4];
       stk[sp++] = id;
       while (sp) {
         const uint32_t x = stk[--sp];
         if (x & BB_LEAF) {
-          const uint32_t q = x & ~BB_LEAF, s0 = unit[q];
-          const float4 f0 = src[s0];
-          const uint32_t nu = PREC_UNITS(__float_as_uint(f0.w), __float_as_uint(f0.z));
-          for (uint32_t u = 0; u < nu; u++) dst[unit_base + u_off + u] = src[s0 + u];
-          u_off += nu;
-          cnt++;
+          uint32_t q = x & ~BB_LEAF, j = cnt++;
+          for (; j > 0 && ps[j - 1] > q; j--) ps[j] = ps[j - 1];
+          ps[j] = q;
         } else {
           const uint2 c = bch[x];
           stk[sp++] = c.y;
           stk[sp++] = c.x;
         }
+      }
+      for (uint32_t t = 0; t < cnt; t++) {
+        const uint32_t s0 = unit[ps[t]];
+        const float4 f0 = src[s0];
+        const uint32_t nu = PREC_UNITS(__float_as_uint(f0.w), __float_as_uint(f0.z));
+        for (uint32_t u = 0; u < nu; u++) dst[unit_base + u_off + u] = src[s0 + u];
+        u_off += nu;
       }
       cw[k] = CHILD_LEAF | ((cnt - 1u) << 28) | (unit_base + u0);
     } else {

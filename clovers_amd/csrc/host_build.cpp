@@ -446,9 +446,15 @@ struct DpCollapser {
     cut(b.nodes[n].left, j, parts);
     cut(b.nodes[n].right, k - j, parts);
   }
+  /* a leaf's records in scene order: the sides of a Boxy that share a leaf
+   * are then tested in the reference's side order (boxy.rs:87-104), which
+   * decides ties at its edges */
   uint32_t leaf_word(int n) {
     const uint32_t off = (uint32_t)(records.size() / 4);
-    for (uint32_t i = 0; i < count[n]; i++) write_record(records, prims[b.perm[first[n] + i]], ext);
+    uint32_t idx[LEAF_MAX_PRIMS];
+    for (uint32_t i = 0; i < count[n]; i++) idx[i] = b.perm[first[n] + i];
+    std::sort(idx, idx + count[n]);
+    for (uint32_t i = 0; i < count[n]; i++) write_record(records, prims[idx[i]], ext);
     return CHILD_LEAF | ((count[n] - 1u) << 28) | off;
   }
   /* the wide node of binary node n (a leaf root becomes a node with one leaf child) */
