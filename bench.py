@@ -192,6 +192,13 @@ def run_prism(args):
     cg = r.counters()
     achieved = cg["guide_queries"] * GUIDE_FLOP / (cg["guide_infer_ms"] * 1e-3) / 1e12 if cg["guide_infer_ms"] else 0.0
     r.close()
+    # the plain GPU method's throughput beside the CPU reference on this scene
+    plain_rate = W * H * spps[-1] / (t_plain * 1e-3) / 1e6
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(synth.prism(), dict(w, samples=spps[-1]), args.cpu_seconds)
+        log("cpu baseline %.3f Msamples/s on %d threads" % (cpu["value"], cpu["cores"]))
     line = {
         "metric": "time-to-RMSE vs oracle, guided vs plain", "value": round(ratio, 3),
         "unit": "x (plain device time / guided device time at equal RMSE)", "n_gpus": 1,
@@ -210,7 +217,9 @@ def run_prism(args):
                      "infer_ms": round(cg["guide_infer_ms"], 3), "train_ms": round(cg["guide_train_ms"], 3),
                      "train_records": cg["guide_records"], "train_rounds": cg["guide_rounds"],
                      "render_ms": round(cg["render_ms"], 3)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
+        "plain_gpu_msamples_per_s": round(plain_rate, 1),
+        "plain_gpu_over_cpu": round(plain_rate / cpu["value"], 1) if cpu else None,
     }
     print(json.dumps(line), flush=True)
 
