@@ -16,15 +16,15 @@
  *      side, the one whose union box has the least surface area; mutual
  *      nearest pairs merge into a new binary node; survivors are compacted in
  *      order, until one cluster is left.  Each new node gets its primitive
- *      count, record size and recursive SAH cost from its two children, and
- *      is marked a leaf when it holds at most 2 primitives, or at most 4 and a
- *      leaf costs no more than the split (traversal step = 0.3 tests);
- *   3. collapse into 4-wide nodes one level at a time: every binary node of a
- *      level opens its largest-area interior child until it has 4 children;
- *      node indices are allocated by prefix sums per level, so the array is
- *      breadth-first (the layout the LDS-cached traversal relies on) and the
- *      result is deterministic.  Leaf records are copied into place in the
- *      same pass.
+ *      count, record size and the decisions of the cost-optimal collapse (the
+ *      host builder's: D(n, k), the least wide-tree cost of its subtree as at
+ *      most k parts, from its two children's, which are final by then);
+ *   3. collapse into 4-wide nodes one level at a time: every wide node takes
+ *      the parts its binary node's decisions give (subtrees of up to 8
+ *      primitives may be leaves); node indices are allocated by prefix sums
+ *      per level, so the array is breadth-first (the layout the LDS-cached
+ *      traversal relies on) and the result is deterministic.  Leaf records
+ *      are copied into place, in scene order, in the same pass.
  *
  * Everything after the host's per-primitive record packing runs on the GPU;
  * the host reads back one counter per clustering iteration and per level.

@@ -280,13 +280,36 @@ __global__ __launch_bounds__(256) void k_pl_flags(uint32_t cnt, const uint32_t *
   fl[i] = keep | (merge << 32);
 }
 
+/* wide-tree cost of a primitive test and of a 4-wide node visit, per unit of
+ * surface area (the host collapse uses the same) */
+#define DP_C_PRIM 1.0f
+#define DP_C_NODE 1.0f
+
+/* D(c, 1..4) of a child: a primitive is a one-primitive leaf in every case */
+__device__ __forceinline__ void child_D(uint32_t id, float area, const float4 *__restrict__ bD, float d[4]) {
+  if (id & BB_LEAF) {
+    d[0] = d[1] = d[2] = d[3] = DP_C_PRIM * area;
+  } else {
+    const float4 v = bD[id];
+    d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+  }
+}
+
 /* merge mutual pairs into new binary nodes (numbered node0 + merge prefix) and
- * compact the survivors in order; pre = exclusive prefix of the flags */
+ * compact the survivors in order; pre = exclusive prefix of the flags.  Each
+ * new node gets its count, record units and the cost-optimal collapse's
+ * decisions: D(n, k) the least wide-tree cost of its subtree as at most k
+ * parts, F(n) = min(a leaf of at most LEAF_MAX_PRIMS primitives, a wide node
+ * W(n) = C_NODE A(n) + min_j D(left, j) + D(right, 4 - j)), and
+ * D(n, k) = min(F(n), min_j D(left, j) + D(right, k - j)).  bdec holds the j
+ * of W (bits 0-1), the j of D(n, 2..4) (bits 2-7, 0 = F whole) and the leaf
+ * flag (bit 8). */
 __global__ __launch_bounds__(256) void k_pl_compact(uint32_t cnt, const uint32_t *__restrict__ nn,
                                                     const uint64_t *__restrict__ pre, const float4 *__restrict__ cbA,
                                                     const uint32_t *__restrict__ cidA, float4 *__restrict__ cbB,
                                                     uint32_t *__restrict__ cidB, uint32_t node0, float4 *__restrict__ bbox,
                                                     uint2 *__restrict__ bch, uint4 *__restrict__ bmeta,
+                                                    float4 *__restrict__ bD, uint32_t *__restrict__ bdec,
                                                     const uint32_t *__restrict__ unit, const float4 *__restrict__ src) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= cnt) return;
@@ -305,13 +328,46 @@ __global__ __launch_bounds__(256) void k_pl_compact(uint32_t cnt, const uint32_t
     const float4 ulo = bb_min(lo, lo2), uhi = bb_max(hi, hi2);
     const float A = bb_area(ulo, uhi), Aa = bb_area(lo, hi), Ab = bb_area(lo2, hi2);
     const uint32_t count = ma.count + mb.count;
-    const float split = A > 0.0f ? 0.3f + (Aa * ma.cost + Ab * mb.cost) / A : 0.3f + ma.cost + mb.cost;
-    const bool leafy = count <= 2u || (count <= 4u && (float)count <= split);
-    const float cost = leafy ? (float)count : split;
+    float da[4], db[4], dn[4];
+    child_D(a, Aa, bD, da);
+    child_D(b, Ab, bD, db);
+    float bw = FLT_MAX;
+    uint32_t dec = 1u;
+#pragma unroll
+    for (int jj = 1; jj <= 3; jj++) {
+      const float c = da[jj - 1] + db[3 - jj];
+      if (c < bw) {
+        bw = c;
+        dec = (uint32_t)jj;
+      }
+    }
+    const float W = DP_C_NODE * A + bw;
+    const float L = count <= LEAF_MAX_PRIMS ? DP_C_PRIM * A * (float)count : FLT_MAX;
+    const bool leafy = L <= W;
+    const float F = leafy ? L : W;
+    dn[0] = F;
+#pragma unroll
+    for (int k = 2; k <= 4; k++) {
+      float dk = F;
+      uint32_t sk = 0;
+#pragma unroll
+      for (int jj = 1; jj < k; jj++) {
+        const float c = da[jj - 1] + db[k - jj - 1];
+        if (c < dk) {
+          dk = c;
+          sk = (uint32_t)jj;
+        }
+      }
+      dn[k - 1] = dk;
+      dec |= sk << (2 * (k - 1));
+    }
+    dec |= leafy ? 0x100u : 0u;
     bbox[2u * node] = ulo;
     bbox[2u * node + 1u] = uhi;
     bch[node] = make_uint2(a, b);
-    bmeta[node] = make_uint4(count, ma.units + mb.units, __float_as_uint(cost), leafy ? 1u : 0u);
+    bmeta[node] = make_uint4(count, ma.units + mb.units, __float_as_uint(F), leafy ? 1u : 0u);
+    bD[node] = make_float4(dn[0], dn[1], dn[2], dn[3]);
+    bdec[node] = dec;
     lo = ulo;
     hi = uhi;
     id = node;
@@ -326,42 +382,40 @@ __device__ __forceinline__ bool is_leafy(uint32_t id, const uint4 *__restrict__ 
   return (id & BB_LEAF) || bmeta[id].w;
 }
 
-/* the children of one 4-wide node: open the largest-area interior child until
- * there are 4; count the interior ones (next level) and the leaves' records */
-__global__ __launch_bounds__(256) void k_cl_plan(uint32_t m, const uint32_t *__restrict__ F, const float4 *__restrict__ bbox,
-                                                 const uint2 *__restrict__ bch, const uint4 *__restrict__ bmeta,
+/* the children of one 4-wide node: the cost-optimal collapse's parts of its
+ * binary node (read back from the merge decisions); count the interior ones
+ * (next level) and the leaves' records */
+__global__ __launch_bounds__(256) void k_cl_plan(uint32_t m, const uint32_t *__restrict__ F, const uint2 *__restrict__ bch,
+                                                 const uint4 *__restrict__ bmeta, const uint32_t *__restrict__ bdec,
                                                  const uint32_t *__restrict__ unit, const float4 *__restrict__ src,
                                                  uint4 *__restrict__ plan, uint64_t *__restrict__ cntv) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= m) return;
   const uint32_t f = F[i];
   uint32_t ch[4] = {BB_NONE, BB_NONE, BB_NONE, BB_NONE};
-  int nc;
+  int nc = 0;
   if (is_leafy(f, bmeta)) { /* only a root can be a leaf */
-    ch[0] = f;
-    nc = 1;
+    ch[nc++] = f;
   } else {
+    /* split (node, k) pairs: k parts of node's subtree, as D(node, k) chose */
+    uint32_t sn[8];
+    int sk[8], sp = 0;
+    const uint32_t w = bdec[f] & 3u;
     const uint2 c = bch[f];
-    ch[0] = c.x;
-    ch[1] = c.y;
-    nc = 2;
-    while (nc < 4) {
-      int pick = -1;
-      float best = -1.0f;
-      for (int k = 0; k < nc; k++) {
-        if (is_leafy(ch[k], bmeta)) continue;
-        const float A = bb_area(bbox[2u * ch[k]], bbox[2u * ch[k] + 1u]);
-        if (A > best) {
-          best = A;
-          pick = k;
-        }
+    sn[sp] = c.y; sk[sp] = 4 - (int)w; sp++;
+    sn[sp] = c.x; sk[sp] = (int)w; sp++;
+    while (sp > 0) {
+      sp--;
+      const uint32_t n = sn[sp];
+      const int k = sk[sp];
+      const uint32_t j = ((n & BB_LEAF) || k == 1) ? 0u : (bdec[n] >> (2 * (k - 1))) & 3u;
+      if (j == 0u) {
+        if (nc < 4) ch[nc++] = n;
+      } else {
+        const uint2 c2 = bch[n];
+        sn[sp] = c2.y; sk[sp] = k - (int)j; sp++;
+        sn[sp] = c2.x; sk[sp] = (int)j; sp++;
       }
-      if (pick < 0) break;
-      const uint2 c2 = bch[ch[pick]];
-      for (int k = nc; k > pick + 1; k--) ch[k] = ch[k - 1];
-      ch[pick] = c2.x;
-      ch[pick + 1] = c2.y;
-      nc++;
     }
   }
   uint32_t nint = 0, units = 0;
@@ -410,10 +464,11 @@ __global__ __launch_bounds__(256) void k_cl_emit(uint32_t m, const uint4 *__rest
     lx[k] = lo.x; ly[k] = lo.y; lz[k] = lo.z;
     hx[k] = hi.x; hy[k] = hi.y; hz[k] = hi.z;
     if (is_leafy(id, bmeta)) {
-      /* the leaf's primitives (at most 4), copied in scene order: the sides of
-       * a Boxy that share a leaf are tested in the reference's side order */
+      /* the leaf's primitives (at most LEAF_MAX_PRIMS), copied in scene order:
+       * the sides of a Boxy that share a leaf are tested in the reference's
+       * side order */
       const uint32_t u0 = u_off;
-      uint32_t stk[8], sp = 0, cnt = 0, ps[4];
+      uint32_t stk[LEAF_MAX_PRIMS + 1], sp = 0, cnt = 0, ps[LEAF_MAX_PRIMS];
       stk[sp++] = id;
       while (sp) {
         const uint32_t x = stk[--sp];
@@ -494,9 +549,10 @@ int device_build_bvh(hipStream_t st, const DeviceBvhInput &in, Bvh4Node *nodes, 
     struct P {
       uint32_t *k0, *h0, *v0, *k1, *h1, *v1, *hist, *nn, *cid0, *cid1, *F0, *F1;
       uint64_t *bsum, *tot, *fl;
-      float4 *cb0, *cb1, *bbox;
+      float4 *cb0, *cb1, *bbox, *bD;
       uint2 *bch;
       uint4 *bmeta, *plan;
+      uint32_t *bdec;
     } p;
     p.k0 = c.take<uint32_t>(n);
     p.h0 = c.take<uint32_t>(n);
@@ -519,6 +575,8 @@ int device_build_bvh(hipStream_t st, const DeviceBvhInput &in, Bvh4Node *nodes, 
     p.bch = c.take<uint2>(n);
     p.bmeta = c.take<uint4>(n);
     p.plan = c.take<uint4>(n);
+    p.bD = c.take<float4>(n);
+    p.bdec = c.take<uint32_t>(n);
     return p;
   };
   Carve sizing;
@@ -584,7 +642,7 @@ int device_build_bvh(hipStream_t st, const DeviceBvhInput &in, Bvh4Node *nodes, 
       BBCHK(scan<uint64_t>(st, P.fl, cnt, P.bsum, P.tot));
       hipLaunchKernelGGL(k_pl_compact, dim3(gc), dim3(256), 0, st, cnt, (const uint32_t *)P.nn,
                          (const uint64_t *)P.fl, (const float4 *)cbA, (const uint32_t *)cidA, cbB, cidB, node0, P.bbox,
-                         P.bch, P.bmeta, in.unit, in.src);
+                         P.bch, P.bmeta, P.bD, P.bdec, in.unit, in.src);
       BBCHK(hipGetLastError());
       uint64_t tot = 0;
       BBCHK(hipMemcpyAsync(&tot, P.tot, 8, hipMemcpyDeviceToHost, st));
@@ -617,8 +675,8 @@ int device_build_bvh(hipStream_t st, const DeviceBvhInput &in, Bvh4Node *nodes, 
         goto done;
       }
       const uint32_t gm = cdiv(m, 256);
-      hipLaunchKernelGGL(k_cl_plan, dim3(gm), dim3(256), 0, st, m, (const uint32_t *)F, (const float4 *)P.bbox,
-                         (const uint2 *)P.bch, (const uint4 *)P.bmeta, in.unit, in.src, P.plan, P.fl);
+      hipLaunchKernelGGL(k_cl_plan, dim3(gm), dim3(256), 0, st, m, (const uint32_t *)F, (const uint2 *)P.bch,
+                         (const uint4 *)P.bmeta, (const uint32_t *)P.bdec, in.unit, in.src, P.plan, P.fl);
       BBCHK(hipGetLastError());
       BBCHK(scan<uint64_t>(st, P.fl, m, P.bsum, P.tot));
       hipLaunchKernelGGL(k_cl_emit, dim3(gm), dim3(256), 0, st, m, (const uint4 *)P.plan, (const uint64_t *)P.fl,
