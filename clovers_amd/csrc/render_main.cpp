@@ -11,7 +11,9 @@
  * (render.rs:52-62); `random` is the only sampler (blue is reported as
  * unsupported).  The output path has no extension, as in the reference; the
  * default is renders/<unix time>.  Mesh paths in the scene file are relative
- * to the working directory, or to --mesh-root.
+ * to the working directory, or to --mesh-root.  --guiding and --builder
+ * (the scene BVH built on the host or on the GPU) have no reference
+ * counterpart.
  */
 #include "../../include/clovers_hip.h"
 
@@ -32,7 +34,8 @@ int usage(const char *msg) {
           "usage: clovers-render -i <scene.json> [-o <output without extension>] [-w 1024] [-h 1024]\n"
           "                      [-s 64] [-d 64] [-m path-tracing|normal-map|bvh-test-count|primitive-test-count]\n"
           "                      [--sampler random] [--bvh sah|lam]\n"
-          "                      [-f png [exr]] [--seed N] [--guiding] [--mesh-root DIR] [--device N]\n");
+          "                      [-f png [exr]] [--seed N] [--guiding] [--mesh-root DIR] [--device N]\n"
+          "                      [--builder host|device]\n");
   return 2;
 }
 
@@ -49,7 +52,7 @@ bool read_text(const std::string &path, std::string *out) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  std::string input, output, mode = "path-tracing", sampler = "random", bvh = "sah", mesh_root;
+  std::string input, output, mode = "path-tracing", sampler = "random", bvh = "sah", mesh_root, builder = "host";
   std::vector<std::string> formats;
   uint32_t width = 1024, height = 1024, samples = 64, max_depth = 64, guiding = 0;
   uint64_t seed = 1;
@@ -80,6 +83,7 @@ int main(int argc, char **argv) {
     else if (a == "--guiding") guiding = 1;
     else if (a == "--mesh-root") mesh_root = val();
     else if (a == "--device") device = (int)u32();
+    else if (a == "--builder") builder = val();
     else if (a == "-f" || a == "--formats") {
       formats.push_back(val());
       while (i + 1 < argc && argv[i + 1][0] != '-') formats.push_back(argv[++i]);
@@ -95,6 +99,7 @@ int main(int argc, char **argv) {
   else return usage("unknown mode");
   if (sampler != "random") return usage("only the random sampler is supported");
   if (bvh != "sah" && bvh != "lam") return usage("--bvh is sah or lam");
+  if (builder != "host" && builder != "device") return usage("--builder is host or device");
   if (formats.empty()) formats.push_back("png");
   for (const std::string &f : formats)
     if (f != "png" && f != "exr") return usage("formats are png and exr");
@@ -120,7 +125,8 @@ int main(int argc, char **argv) {
     return 1;
   }
   uint32_t sid = 0;
-  if ((st = clv_scene_upload(ctx, clv_scene_file_desc(sf), &sid)) != CLV_OK) {
+  if ((st = clv_scene_upload_ex(ctx, clv_scene_file_desc(sf), builder == "device" ? CLV_BUILD_DEVICE : CLV_BUILD_HOST,
+                                &sid)) != CLV_OK) {
     fprintf(stderr, "error: scene upload: %s\n", clv_last_error(ctx));
     clv_destroy(ctx);
     clv_scene_file_free(sf);

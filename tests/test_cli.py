@@ -26,6 +26,7 @@ def run(*args, **kw):
     (("-i", "x.json", "--sampler", "blue"), 2),
     (("-i", "x.json", "-f", "jpg"), 2),
     (("-i", "x.json", "-w", "abc"), 2),
+    (("-i", "x.json", "--builder", "gpu"), 2),
     (("-i", "/nonexistent/scene.json"), 1),
 ])
 def test_arguments(args, code):
@@ -68,3 +69,9 @@ def test_render_scene_file(tmp_path, renderer):
     want = renderer.render_debug(sid, abi.render_opts(64, 48, 1, 1, seed=3), abi.MODE_NORMAL_MAP)
     renderer.free(sid)
     assert np.array_equal(nm, want)
+    # the scene BVH built on the GPU: the same image (the room has no ties between equally distant hits)
+    r = run("-i", p, "-w", 96, "-h", 64, "-s", 8, "-d", 8, "-f", "exr", "-o", tmp_path / "d", "--seed", 3,
+            "--builder", "device")
+    assert r.returncode == 0, r.stderr
+    dimg, _ = read_exr(str(tmp_path / "d") + ".exr")
+    assert np.array_equal(dimg, img)

@@ -128,3 +128,11 @@ def test_device_build_repeated_uploads(renderer):
     finally:
         for sid in (d1, d2, h1):
             renderer.free(sid)
+
+
+def test_unknown_builder_is_rejected(renderer):
+    import ctypes as C
+    desc, keep = synth.cornell().as_ctypes()
+    sid = C.c_uint32()
+    st = renderer.lib.clv_scene_upload_ex(renderer.h, C.byref(desc), 7, C.byref(sid))
+    assert st == abi.EINVAL
