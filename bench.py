@@ -340,6 +340,11 @@ def main():
         tuning = r.autotune(jobs)
         tune_ms = (time.perf_counter() - t0) * 1e3
         log("autotune %.0f ms: %s" % (tune_ms, tuning))
+    # setup, like the scene upload and the autotuner: the library captures a
+    # repeated render as a hipGraph on its second identical call, so two
+    # renders here leave every warmup and timed step a graph replay
+    for i in range(2):
+        r.render_batch(jobs)
     for i in range(args.warmup):
         r.render_batch(jobs)
         log("warmup %d done" % i)
