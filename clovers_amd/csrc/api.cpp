@@ -42,6 +42,9 @@ static_assert(CLV_TRACE_SIMPLE == TRACE_SIMPLE && CLV_TRACE_PERSISTENT == TRACE_
 
 namespace {
 
+/* results up to this many bytes come back through pinned staging memory */
+#define PIN_STAGE_MAX (16u << 20)
+
 /* one growable device array of the scene arena */
 struct DevArray {
   char *p = nullptr;
@@ -117,6 +120,10 @@ struct clv_ctx {
   float *out = nullptr, *out_sq = nullptr;
   uint32_t *ids = nullptr;
   float *ts = nullptr;
+  /* pinned staging for results of small renders: a DMA into pinned memory and
+   * a host copy beat the runtime's pageable path when the images are small */
+  char *pin = nullptr;
+  size_t cap_pin = 0;
   /* job and pass-item tables of the current render */
   DevJob *jobs = nullptr;
   uint32_t cap_jobs = 0;
@@ -756,16 +763,47 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   }
   HIPCHK(c, hipEventRecord(c->events[1], c->stream));
   HIPCHK(c, hipGetLastError());
-  for (uint32_t k = 0; k < n; k++) {
-    const size_t off = 3 * (size_t)dj[k].pixel_base, cnt = 3 * (size_t)dj[k].width * dj[k].height;
-    HIPCHK(c, hipMemcpyAsync(jobs[k].xyz_out, c->out + off, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->stream));
-    if (jobs[k].xyz_sq_out)
-      HIPCHK(c, hipMemcpyAsync(jobs[k].xyz_sq_out, c->out_sq + off, sizeof(float) * cnt, hipMemcpyDeviceToHost,
-                               c->stream));
+  /* results: through the pinned staging buffer when they are small (one DMA
+   * per array, then host copies), else straight into the caller's memory */
+  const size_t img_bytes = sizeof(float) * 3 * (size_t)pixels;
+  const bool any_sq_out = [&] {
+    for (uint32_t k = 0; k < n; k++)
+      if (jobs[k].xyz_sq_out) return true;
+    return false;
+  }();
+  const size_t stage_bytes = 64 + img_bytes * (any_sq_out ? 2 : 1);
+  bool staged = stage_bytes <= PIN_STAGE_MAX;
+  if (staged && stage_bytes > c->cap_pin) {
+    if (c->pin) (void)hipHostFree(c->pin);
+    c->pin = nullptr;
+    c->cap_pin = 0;
+    if (hipHostMalloc((void **)&c->pin, stage_bytes, hipHostMallocDefault) == hipSuccess) c->cap_pin = stage_bytes;
+    else staged = false;
   }
   uint64_t stats[8];
-  HIPCHK(c, hipMemcpyAsync(stats, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (staged) {
+    HIPCHK(c, hipMemcpyAsync(c->pin, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pin + 64, c->out, img_bytes, hipMemcpyDeviceToHost, c->stream));
+    if (any_sq_out)
+      HIPCHK(c, hipMemcpyAsync(c->pin + 64 + img_bytes, c->out_sq, img_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memcpy(stats, c->pin, sizeof(stats));
+    for (uint32_t k = 0; k < n; k++) {
+      const size_t off = 3 * (size_t)dj[k].pixel_base, cnt = 3 * (size_t)dj[k].width * dj[k].height;
+      memcpy(jobs[k].xyz_out, c->pin + 64 + sizeof(float) * off, sizeof(float) * cnt);
+      if (jobs[k].xyz_sq_out) memcpy(jobs[k].xyz_sq_out, c->pin + 64 + img_bytes + sizeof(float) * off, sizeof(float) * cnt);
+    }
+  } else {
+    for (uint32_t k = 0; k < n; k++) {
+      const size_t off = 3 * (size_t)dj[k].pixel_base, cnt = 3 * (size_t)dj[k].width * dj[k].height;
+      HIPCHK(c, hipMemcpyAsync(jobs[k].xyz_out, c->out + off, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->stream));
+      if (jobs[k].xyz_sq_out)
+        HIPCHK(c, hipMemcpyAsync(jobs[k].xyz_sq_out, c->out_sq + off, sizeof(float) * cnt, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(stats, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   float ms = 0.0f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->events[0], c->events[1]));
   clv_counters &cn = c->counters;
@@ -878,6 +916,7 @@ void clv_destroy(clv_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->pin) (void)hipHostFree(c->pin);
   for (auto &kv : c->scenes) delete kv.second;
   for (auto &a : c->arena) dfree(a.p);
   free_paths(c);
