@@ -164,6 +164,8 @@ def run_prism(args):
         r.set_tuning(guide_rounds=args.guide_rounds)
     if args.guide_train_spp:
         r.set_tuning(guide_train_spp=args.guide_train_spp)
+    if args.guide_lr:
+        r.set_tuning(guide_lr=args.guide_lr)
     tuning = r.autotune([(sid, abi.render_opts(W, H, spps[-1], D, seed=1))])
     r.set_tuning(use_graph=0)  # every render here is a different one: nothing to replay
     log("autotune: %s" % tuning)
@@ -243,6 +245,7 @@ def main():
                          "caustic paths, so one render's RMSE is noisy)")
     ap.add_argument("--guide-alpha", type=float, default=0.0, help="prism workload: override the guide's share")
     ap.add_argument("--guide-rounds", type=int, default=0, help="prism workload: override the training rounds")
+    ap.add_argument("--guide-lr", type=float, default=0.0, help="prism workload: override the Adam learning rate")
     ap.add_argument("--guide-train-spp", type=int, default=0,
                     help="prism workload: override the samples per pixel between training rounds")
     ap.add_argument("--spp", type=int, default=0,

@@ -98,6 +98,7 @@ struct GuideState {
   uint32_t rounds = 2;
   uint32_t credit = 1;       /* clv_tuning.guide_credit */
   uint32_t step = GUIDE_STEP; /* records per Adam step */
+  uint32_t epochs = 1;        /* passes over a round's records */
 };
 
 }  // namespace
@@ -510,11 +511,13 @@ void guide_train_round(clv_ctx *c, uint32_t paths, uint32_t stamp, float *grad_o
   launch_guide_collect(c->stream, g.rec, g.recL, paths * GD_SLOTS, g.bcount, g.bweight, g.totals, g.list, cap,
                        (unsigned long long *)c->stats + 4);
   const uint32_t step = grad_out ? cap : g.step;
-  for (uint32_t b = 0; b < cap; b += step) {
-    if (!grad_out) g.t++;
-    launch_guide_step(c->stream, g.rec, g.recL, g.list, g.totals, cap, b, std::min(step, cap - b), wclip, g.P, g.M, g.V,
-                      g.Wh, g.partial, g.lr, std::max<uint32_t>(g.t, 1), grad_out);
-  }
+  const uint32_t epochs = grad_out ? 1u : std::max<uint32_t>(1u, g.epochs);
+  for (uint32_t e = 0; e < epochs; e++)
+    for (uint32_t b = 0; b < cap; b += step) {
+      if (!grad_out) g.t++;
+      launch_guide_step(c->stream, g.rec, g.recL, g.list, g.totals, cap, b, std::min(step, cap - b), wclip, g.P, g.M,
+                        g.V, g.Wh, g.partial, g.lr, std::max<uint32_t>(g.t, 1), grad_out);
+    }
 }
 
 /* the batch render behind clv_render and clv_render_batch */
@@ -884,6 +887,7 @@ clv_status clv_create(int hip_device, clv_ctx **out) {
   c->trace_blocks = trace_grid_blocks();
   if (const char *e = getenv("CLV_TRACE_BLOCKS")) c->trace_blocks = (uint32_t)strtoul(e, nullptr, 10);
   if (const char *e = getenv("CLV_GUIDE_STEP")) c->guide.step = std::max<uint32_t>(64u, (uint32_t)strtoul(e, nullptr, 10) & ~63u);
+  if (const char *e = getenv("CLV_GUIDE_EPOCHS")) c->guide.epochs = std::max<uint32_t>(1u, (uint32_t)strtoul(e, nullptr, 10));
   if (const char *e = getenv("CLV_TRACE_KERNEL"))
     c->trace_variant = strcmp(e, "persistent") == 0 ? TRACE_PERSISTENT
                        : strcmp(e, "ww") == 0       ? TRACE_WW
