@@ -110,6 +110,7 @@ struct clv_ctx {
   std::map<uint32_t, SceneRes *> scenes;
   uint32_t next_scene = 1;
   DevArray arena[AR_COUNT];
+  uint32_t tile_order = 1;   /* render passes lay each sample's slots out in 8x8 pixel tiles */
   /* path state */
   uint32_t cap_paths = 0;
   PathBuf pb;                /* the view of even bounces; odd bounces swap the two entry sets */
@@ -559,6 +560,7 @@ clv_status render_jobs(clv_ctx *c, const clv_job *jobs, uint32_t n) {
   HIPCHK(c, hipSetDevice(c->device));
   PassPlan pl;
   plan_passes(dj, soff, c->target_paths, std::max<uint32_t>(1, c->guide.train_spp), c->guide.rounds, &pl);
+  for (DevItem &it : pl.items) it.order = c->tile_order;
   const uint32_t passes = (uint32_t)pl.paths.size();
   CHK(ensure_paths(c, pl.max_paths));
   CHK(ensure_pixels(c, (uint32_t)pixels));
@@ -887,6 +889,7 @@ clv_status clv_create(int hip_device, clv_ctx **out) {
   c->trace_blocks = trace_grid_blocks();
   if (const char *e = getenv("CLV_TRACE_BLOCKS")) c->trace_blocks = (uint32_t)strtoul(e, nullptr, 10);
   if (const char *e = getenv("CLV_GUIDE_STEP")) c->guide.step = std::max<uint32_t>(64u, (uint32_t)strtoul(e, nullptr, 10) & ~63u);
+  if (const char *e = getenv("CLV_TILE_ORDER")) c->tile_order = strcmp(e, "0") == 0 ? 0u : 1u;
   if (const char *e = getenv("CLV_GUIDE_EPOCHS")) c->guide.epochs = std::max<uint32_t>(1u, (uint32_t)strtoul(e, nullptr, 10));
   if (const char *e = getenv("CLV_TRACE_KERNEL"))
     c->trace_variant = strcmp(e, "persistent") == 0 ? TRACE_PERSISTENT

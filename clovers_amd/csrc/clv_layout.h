@@ -177,7 +177,8 @@ typedef struct {
   uint32_t samples;       /* samples per pixel in this pass */
   uint32_t sample_base;   /* absolute index of the item's first sample */
   uint32_t res_base;      /* first resolve thread of the item (prefix sum of pixels) */
-  uint32_t pad[2];
+  uint32_t order;         /* pixel order of the item's slots within a sample: 0 row-major, 1 8x8 tiles */
+  uint32_t pad;
 } DevItem;
 
 #endif

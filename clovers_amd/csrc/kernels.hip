@@ -481,6 +481,22 @@ __device__ __forceinline__ uint32_t find_entry(const T *tab, uint32_t n, uint32_
 
 /* camera sample of sampler/random.rs:20-32 and Camera::get_ray (camera.rs:103-126);
  * path slot i belongs to the pass item whose range holds it */
+/* row-major pixel index (row 0 = top) of the q-th slot of a sample in tile
+ * order: bands of 8 rows top to bottom, 8x8 tiles left to right within a
+ * band, row-major within a tile; the last band and the last tile of a band
+ * are shorter / narrower when H or W is not a multiple of 8.  A bijection of
+ * [0, W*H): a wave's 64 rays cover one tile instead of 64 pixels of a row. */
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t q, uint32_t W, uint32_t H) {
+  const uint32_t band = q / (8u * W);
+  const uint32_t q1 = q - band * 8u * W;
+  const uint32_t hb = min(8u, H - 8u * band);
+  const uint32_t tc = q1 / (8u * hb);
+  const uint32_t q2 = q1 - tc * 8u * hb;
+  const uint32_t tw = min(8u, W - 8u * tc);
+  const uint32_t yi = q2 / tw;
+  return (8u * band + yi) * W + 8u * tc + (q2 - yi * tw);
+}
+
 __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenParams gp) {
   uint32_t n = gp.paths;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -489,6 +505,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene sc, PathBuf pb, GenPa
     const uint32_t W = jb.width, H = jb.height;
     uint32_t k = i - it.path_base;
     uint32_t j = k / it.pixels, o = k - j * it.pixels;
+    if (it.order) o = tile_pixel(o, W, H);
     uint32_t r = o / W, x = o - r * W;
     uint32_t y = H - 1u - r;
     uint32_t sample = it.sample_base + j;
@@ -1529,9 +1546,11 @@ __global__ __launch_bounds__(256) void k_resolve(PathBuf pb, ResolveParams rp, c
   const DevJob *jobs = rp.jobs;
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < rp.threads; g += gridDim.x * blockDim.x) {
     const DevItem it = rp.items[find_entry(rp.items, rp.n_items, g, &DevItem::res_base)];
-    const uint32_t o = g - it.res_base;
-    const bool fp64 = jobs[it.job].fp64 != 0;
-    const uint32_t a = 3u * (jobs[it.job].pixel_base + o);
+    const uint32_t q = g - it.res_base; /* slot within a sample */
+    const DevJob &jb = jobs[it.job];
+    const uint32_t o = it.order ? tile_pixel(q, jb.width, jb.height) : q;
+    const bool fp64 = jb.fp64 != 0;
+    const uint32_t a = 3u * (jb.pixel_base + o);
     float a32[3];
     double a64[3], sq[3];
     for (int k = 0; k < 3; k++) {
@@ -1540,7 +1559,7 @@ __global__ __launch_bounds__(256) void k_resolve(PathBuf pb, ResolveParams rp, c
       else a32[k] = rp.acc32[a + k];
     }
     for (uint32_t j = 0; j < it.samples; j++) {
-      uint32_t i = it.path_base + j * it.pixels + o;
+      uint32_t i = it.path_base + j * it.pixels + q;
       float4 L = pb.rad[i];
       uint32_t hero = pb.hero[i];
       float c[3] = {0.0f, 0.0f, 0.0f};
