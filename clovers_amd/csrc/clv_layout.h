@@ -7,9 +7,9 @@
  *              (lo/hi per axis x 4 children) + 4 child words.
  *   prims      variable-size primitive records in float4 units:
  *                sphere 2 x float4 (32 B), moving sphere 3 x float4 (48 B:
- *                centre at time 0, radius, centre_1 - centre_0), triangle
- *                4 x float4 (64 B: corners, plane d, unit normal),
- *                quad 5 x float4 (80 B), constant medium 3 x float4 with a
+ *                centre at time 0, radius, centre_1 - centre_0), quad and
+ *                triangle 5 x float4 (80 B: q, u, v, plane d, unit normal,
+ *                w = n / n.n), constant medium 3 x float4 with a
  *                sphere boundary or 32 x float4 with a box boundary (its six
  *                quad records; media are not in the BVH: each scene's media
  *                records follow its leaf records and every ray tests them
@@ -42,12 +42,12 @@
 
 /* record sizes in float4 units */
 #define PREC_SPHERE 2u
-#define PREC_TRI 4u
+#define PREC_TRI 5u
 #define PREC_QUAD 5u
 #define PREC_MSPHERE 3u
-/* record size by type as a nibble table: PT_SPHERE 2, PT_QUAD 5, PT_TRI 4, PT_MSPHERE 3
+/* record size by type as a nibble table: PT_SPHERE 2, PT_QUAD 5, PT_TRI 5, PT_MSPHERE 3
  * (a medium's size is in its record: see PREC_UNITS) */
-#define PREC_TABLE 0x3452u
+#define PREC_TABLE 0x3552u
 #define PREC_OF(type) ((PREC_TABLE >> ((type) << 2)) & 0xFu)
 
 /* constant medium record (objects/constant_medium.rs):

@@ -237,10 +237,7 @@ void write_record(std::vector<float> &out, const FlatPrim &p, const std::vector<
   } else if (p.type == PT_MSPHERE) { /* [c0, hdr] [r, 0, 0, info] [c1 - c0, 0] */
     float r[12] = {d[0], d[1], d[2], fh, d[3], z, z, fi, d[4], d[5], d[6], z};
     out.insert(out.end(), r, r + 12);
-  } else if (p.type == PT_TRI) {    /* [a, hdr] [b, info] [c, plane d] [n, 0] */
-    float r[16] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], d[12], d[9], d[10], d[11], z};
-    out.insert(out.end(), r, r + 16);
-  } else {                          /* [q, hdr] [u, info] [v, d] [n, area] [w, 0] */
+  } else {                          /* quad and triangle: [q, hdr] [u, info] [v, d] [n, area] [w, 0] */
     float r[20] = {d[0], d[1], d[2], fh, d[3], d[4], d[5], fi, d[6], d[7], d[8], d[12],
                    d[9], d[10], d[11], d[16], d[13], d[14], d[15], z};
     out.insert(out.end(), r, r + 20);
@@ -893,29 +890,17 @@ int build_host_scene(const clv_scene_desc *d, HostScene *hs, std::string *err, b
         bool tri = o.kind != CLV_OBJ_QUAD;
         F3 a = f3(p[0], p[1], p[2]), b = f3(p[3], p[4], p[5]), c = f3(p[6], p[7], p[8]);
         Plane pl;
-        F3 va, vb, vc;  /* triangle corners for the watertight test */
-        if (o.kind == CLV_OBJ_TRIANGLE) {       /* Triangle::from_coordinates */
+        if (o.kind == CLV_OBJ_TRIANGLE)         /* Triangle::from_coordinates */
           pl = make_plane(a, b - a, c - a, true);
-          va = a; vb = b; vc = c;
-        } else {                                /* Quad::new / Triangle::new(q, u, v) */
+        else                                    /* Quad::new / Triangle::new(q, u, v) */
           pl = make_plane(a, b, c, tri);
-          va = a; vb = a + b; vc = a + c;
-        }
         obox = plane_box(pl, tri);
         FlatPrim fp;
         memset(&fp, 0, sizeof(fp));
-        if (tri) {
-          fp.type = PT_TRI;
-          float *q = fp.d;
-          q[0] = va.x; q[1] = va.y; q[2] = va.z;
-          q[3] = vb.x; q[4] = vb.y; q[5] = vb.z;
-          q[6] = vc.x; q[7] = vc.y; q[8] = vc.z;
-          q[9] = pl.n.x; q[10] = pl.n.y; q[11] = pl.n.z;
-          q[12] = pl.d;
-        } else {
-          fp.type = PT_QUAD;
-          quad_payload(&fp, pl);
-        }
+        /* triangles keep the quad's plane record: the device decides inside /
+         * outside with the same plane coordinates as triangle.rs:143-160 */
+        fp.type = tri ? PT_TRI : PT_QUAD;
+        quad_payload(&fp, pl);
         local.push_back(fp);
         if (o.priority && !xf) {
           DevLight L;

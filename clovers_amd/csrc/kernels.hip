@@ -156,7 +156,14 @@ __device__ __forceinline__ bool sphere_t(V c, float r, V o, V d, float tmin, flo
 }
 
 /* Quad::hit (objects/quad.rs:88-132): |n.d| < 1e-4 rejected, inclusive t and (a, b) bounds */
-__device__ __forceinline__ bool quad_t(const float4 *rec, V o, V d, float tmin, float tmax, float &t) {
+/* Quad::hit (quad.rs:88-132) and Triangle::hit (triangle.rs:120-166): the
+ * plane distance, then the plane coordinates (alpha, beta) of the hit point,
+ * inside if both are in [0, 1] (and alpha + beta <= 1 for a triangle,
+ * triangle.rs:229-234), computed with the reference's operations so hits and
+ * distances are the reference's own, including where a ray meets a shared
+ * edge and the rounding puts it outside both triangles */
+template <bool TRI>
+__device__ __forceinline__ bool plane_t(const float4 *rec, V o, V d, float tmin, float tmax, float &t) {
   float4 f0 = rec[0], f1 = rec[1], f2 = rec[2], f3 = rec[3], f4 = rec[4];
   V q = xyz(f0), u = xyz(f1), v = xyz(f2), n = xyz(f3), w = xyz(f4);
   float denom = dot(n, d);
@@ -168,56 +175,11 @@ __device__ __forceinline__ bool quad_t(const float4 *rec, V o, V d, float tmin, 
   float alpha = dot(w, cross(ph, v));
   float beta = dot(w, cross(u, ph));
   if (!(alpha >= 0.0f && alpha <= 1.0f && beta >= 0.0f && beta <= 1.0f)) return false;
+  if (TRI && !(alpha + beta <= 1.0f)) return false;
   t = tt;
   return true;
 }
 
-/* Per-ray constants of the watertight triangle test (Woop, Benthin, Wald 2013) */
-struct WRay {
-  int kx, ky, kz;
-  float sx, sy, sz;
-};
-__device__ __forceinline__ WRay wray(V d) {
-  WRay w;
-  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-  w.kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
-  w.kx = w.kz == 2 ? 0 : w.kz + 1;
-  w.ky = w.kx == 2 ? 0 : w.kx + 1;
-  if (comp(d, w.kz) < 0.0f) { int t = w.kx; w.kx = w.ky; w.ky = t; }
-  float dz = comp(d, w.kz);
-  w.sx = comp(d, w.kx) / dz;
-  w.sy = comp(d, w.ky) / dz;
-  w.sz = 1.0f / dz;
-  return w;
-}
-
-/* Triangle test: the watertight inside test (no gaps or double hits along
- * shared edges) decides the hit; the reference's rules decide the rest --
- * |n.d| < 1e-4 is rejected, and the distance is the reference's plane
- * distance t = (-d - n.o) / (n.d) with inclusive bounds (triangle.rs:129-139),
- * so distances agree with the reference's to the last bit. */
-__device__ __forceinline__ bool tri_t(const float4 *rec, V o, V d, const WRay &wr, float tmin, float tmax, float &t) {
-  float4 f0 = rec[0], f1 = rec[1], f2 = rec[2], f3 = rec[3];
-  V n = xyz(f3);
-  float denom = dot(n, d);
-  if (fabsf(denom) < EPS_RECT) return false;
-  float tt = (-f2.w - dot(n, o)) / denom;
-  if (!(tt >= tmin && tt <= tmax)) return false; /* also rejects NaN */
-  V A = xyz(f0) - o, B = xyz(f1) - o, C = xyz(f2) - o;
-  float Akx = comp(A, wr.kx), Aky = comp(A, wr.ky), Akz = comp(A, wr.kz);
-  float Bkx = comp(B, wr.kx), Bky = comp(B, wr.ky), Bkz = comp(B, wr.kz);
-  float Ckx = comp(C, wr.kx), Cky = comp(C, wr.ky), Ckz = comp(C, wr.kz);
-  float Ax = Akx - wr.sx * Akz, Ay = Aky - wr.sy * Akz;
-  float Bx = Bkx - wr.sx * Bkz, By = Bky - wr.sy * Bkz;
-  float Cx = Ckx - wr.sx * Ckz, Cy = Cky - wr.sy * Ckz;
-  float U = Cx * By - Cy * Bx;
-  float Vv = Ax * Cy - Ay * Cx;
-  float W = Bx * Ay - By * Ax;
-  if ((U < 0.0f || Vv < 0.0f || W < 0.0f) && (U > 0.0f || Vv > 0.0f || W > 0.0f)) return false;
-  if (U + Vv + W == 0.0f) return false;
-  t = tt;
-  return true;
-}
 
 /* RotateY/Translate chain, outermost op first (rotate.rs:97-113, translate.rs:60-66) */
 __device__ __forceinline__ void xform_ray(const Xform &x, V &o, V &d) {
@@ -239,7 +201,7 @@ __device__ __forceinline__ bool box_t(const float4 *sides, V o, V d, float tmin,
   float closest = tmax;
   for (int k = 0; k < 6; k++) {
     float tt;
-    if (quad_t(sides + PREC_QUAD * k, o, d, tmin, closest, tt)) {
+    if (plane_t<false>(sides + PREC_QUAD * k, o, d, tmin, closest, tt)) {
       closest = tt;
       any = true;
     }
@@ -283,25 +245,21 @@ __device__ __forceinline__ bool medium_t(const DevScene &sc, const float4 *rec, 
 __device__ __forceinline__ uint32_t prim_units(uint32_t type) { return PREC_OF(type); }
 
 template <bool HAS_XFORM>
-__device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec, uint32_t hdr, V o, V d, const WRay &wr,
-                                          float time, float tmin, float tmax, float &t) {
+__device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec, uint32_t hdr, V o, V d, float time,
+                                          float tmin, float tmax, float &t) {
   uint32_t type = hdr & PT_TYPE_MASK;
   uint32_t xf = hdr >> PT_XF_SHIFT;
-  WRay w2 = wr;
-  if (HAS_XFORM && xf) {
-    xform_ray(sc.xforms[xf - 1u], o, d);
-    if (type == PT_TRI) w2 = wray(d);
-  }
+  if (HAS_XFORM && xf) xform_ray(sc.xforms[xf - 1u], o, d);
   if (type == PT_SPHERE) {
     float4 f0 = rec[0], f1 = rec[1];
     return sphere_t(xyz(f0), f1.x, o, d, tmin, tmax, t);
   }
-  if (type == PT_TRI) return tri_t(rec, o, d, w2, tmin, tmax, t);
+  if (type == PT_TRI) return plane_t<true>(rec, o, d, tmin, tmax, t);
   if (type == PT_MSPHERE) { /* MovingSphere::hit (moving_sphere.rs:106-160): the sphere at the ray's time */
     float4 f0 = rec[0], f1 = rec[1], f2 = rec[2];
     return sphere_t(xyz(f0) + time * xyz(f2), f1.x, o, d, tmin, tmax, t);
   }
-  return quad_t(rec, o, d, tmin, tmax, t);
+  return plane_t<false>(rec, o, d, tmin, tmax, t);
 }
 
 /* ------------------------------------------------------ traversal --- */
@@ -317,7 +275,6 @@ __device__ __forceinline__ bool test_prim(const DevScene &sc, const float4 *rec,
 /* state of one lane's ray */
 struct Lane {
   V o, d, inv, oi;
-  WRay wr;
   float tb;          /* closest accepted distance so far (the reference starts at f32::MAX, trace.rs:44) */
   float time;        /* the ray's instant in [0, 1): moving spheres are tested where they are then */
   uint32_t best;     /* record offset of the closest hit, MISS_WORD if none */
@@ -337,7 +294,6 @@ __device__ __forceinline__ void lane_start(Lane &L, const PathBuf &pb, uint32_t 
             fabsf(L.d.z) < 1e-20f ? copysignf(1e-20f, L.d.z) : L.d.z);
   L.inv = mk(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
   L.oi = mk(L.o.x * L.inv.x, L.o.y * L.inv.y, L.o.z * L.inv.z);
-  L.wr = wray(L.d);
   L.tb = FLT_MAX;
   L.best = MISS_WORD;
   /* a ray with a NaN component misses at the root, as in the reference, where
@@ -411,8 +367,8 @@ __device__ __forceinline__ uint32_t lane_leaf(Lane &L, const DevScene &sc, bool 
     const float4 *rec = sc.prims + off;
     uint32_t hdr = as_u(rec[0].w);
     float t;
-    bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t)
-                       : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t);
+    bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.time, EPS_ACNE, L.tb, t)
+                       : test_prim<false>(sc, rec, hdr, L.o, L.d, L.time, EPS_ACNE, L.tb, t);
     if (h) {
       L.tb = t;
       L.best = off;
@@ -616,8 +572,8 @@ __device__ __forceinline__ void lprim_at(Lane &L, const DevScene &sc, const floa
                                          uint32_t &units) {
   uint32_t hdr = as_u(rec[0].w);
   float t;
-  bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t)
-                     : test_prim<false>(sc, rec, hdr, L.o, L.d, L.wr, L.time, EPS_ACNE, L.tb, t);
+  bool h = any_xform ? test_prim<true>(sc, rec, hdr, L.o, L.d, L.time, EPS_ACNE, L.tb, t)
+                     : test_prim<false>(sc, rec, hdr, L.o, L.d, L.time, EPS_ACNE, L.tb, t);
   if (h) {
     L.tb = t;
     L.best = off;
@@ -936,16 +892,8 @@ __device__ Hit hit_record(const DevScene &sc, uint32_t off, V o, V d, float time
   } else {
     outward = xyz(rec[3]);
     if (want_uv) {
-      /* (u, v) = the plane coordinates (alpha, beta) of quad.rs:108-128 / triangle.rs:143-160;
-       * a triangle's edge vectors are rebuilt from its stored corners */
-      V q, eu, ev, w;
-      if (type == PT_QUAD) {
-        q = xyz(f0); eu = xyz(f1); ev = xyz(rec[2]); w = xyz(rec[4]);
-      } else {
-        q = xyz(f0); eu = xyz(f1) - q; ev = xyz(rec[2]) - q;
-        V nn = cross(eu, ev);
-        w = nn / dot(nn, nn);
-      }
+      /* (u, v) = the plane coordinates (alpha, beta) of quad.rs:108-128 / triangle.rs:143-160 */
+      const V q = xyz(f0), eu = xyz(f1), ev = xyz(rec[2]), w = xyz(rec[4]);
       V ph = p - q;
       h.u = dot(w, cross(ph, ev));
       h.v = dot(w, cross(eu, ph));

@@ -59,6 +59,26 @@ def test_primary_hit_ids(renderer, name):
     o = abi.render_opts(W, H, 1, depth, seed=1)
     gi, gt = renderer.primary_hits(gpu_scene(renderer, name), o)
     oi, ot = oracle_scene(name).primary_hits(o)
+    check_hits(name, gi, gt, oi, ot)
+
+
+@pytest.mark.parametrize("name,make", [("spheres", synth.spheres), ("mesh 1M", synth.mesh)])
+def test_primary_hit_ids_full_size(renderer, name, make):
+    """the benchmark scenes at the benchmark resolution: the 488-sphere field and
+    the 1,003,520-triangle displaced icosphere, 1920x1080 (2.07M rays each)"""
+    s = make()
+    o = abi.render_opts(1920, 1080, 1, 12, seed=1)
+    sid = renderer.upload(s)
+    try:
+        gi, gt = renderer.primary_hits(sid, o)
+    finally:
+        renderer.free(sid)
+    oi, ot = po.OracleScene(s).primary_hits(o)
+    assert (gi != abi.MISS).mean() > 0.3
+    check_hits(name, gi, gt, oi, ot)
+
+
+def check_hits(name, gi, gt, oi, ot):
     same = gi == oi
     frac = same.mean()
     mism = ~same
