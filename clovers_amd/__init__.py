@@ -259,14 +259,28 @@ class Renderer:
                                         sq.ctypes.data if sq is not None else None))
         return out, sq
 
-    def _jobs(self, jobs, want_sq):
+    @staticmethod
+    def _reuse(buf, shape, what):
+        if not (isinstance(buf, np.ndarray) and buf.dtype == np.float32 and buf.shape == shape
+                and buf.flags.c_contiguous and buf.flags.writeable):
+            raise ValueError("%s must be a writeable C-contiguous float32 array of shape %s" % (what, shape))
+        return buf
+
+    def _jobs(self, jobs, want_sq, outs_in=None):
         arr = (abi.Job * len(jobs))()
         outs = []
+        if outs_in is not None and len(outs_in) != len(jobs):
+            raise ValueError("outs must hold one (xyz, sq) pair per job")
         for k, jb in enumerate(jobs):
             sid, opts = jb[0], jb[1]
             cam = jb[2] if len(jb) > 2 else None
-            out = np.zeros((opts.height, opts.width, 3), np.float32)
-            sq = np.zeros((opts.height, opts.width, 3), np.float32) if want_sq else None
+            shape = (opts.height, opts.width, 3)
+            if outs_in is not None:
+                out = self._reuse(outs_in[k][0], shape, "outs[%d][0]" % k)
+                sq = self._reuse(outs_in[k][1], shape, "outs[%d][1]" % k) if want_sq else None
+            else:
+                out = np.zeros(shape, np.float32)
+                sq = np.zeros(shape, np.float32) if want_sq else None
             arr[k].scene_id = sid
             arr[k].use_camera = 1 if cam is not None else 0
             if cam is not None:
@@ -325,11 +339,13 @@ class Renderer:
         self._check(self.lib.clv_guide_grad(self.h, x.ctypes.data, x.shape[0], g.ctypes.data))
         return g
 
-    def render_batch(self, jobs, want_sq=False):
+    def render_batch(self, jobs, want_sq=False, outs=None):
         """clv_render_batch.  jobs: list of (scene_id, opts) or (scene_id, opts,
         camera) with camera an abi.CameraInit replacing the scene's camera.
-        Returns [(xyz[H, W, 3], sq or None)] in job order."""
-        arr, outs = self._jobs(jobs, want_sq)
+        Returns [(xyz[H, W, 3], sq or None)] in job order.  outs: the list a
+        previous call returned, to be overwritten instead of allocating new
+        arrays (a caller's framebuffers)."""
+        arr, outs = self._jobs(jobs, want_sq, outs)
         self._check(self.lib.clv_render_batch(self.h, arr, len(jobs)))
         return outs
 

@@ -151,3 +151,25 @@ def test_graph_replays_match_eager_renders():
                 for g, w_ in zip(got, ref):
                     assert np.array_equal(g, w_), rep
                 r.primary_hits(a, abi.render_opts(32, 32, 1, 8, seed=rep))  # eager launches between replays
+
+
+def test_render_batch_reuses_caller_arrays(renderer):
+    """outs= overwrites the arrays of a previous call in place with the same bytes
+    a fresh call returns; a wrong shape is refused before anything runs"""
+    sid = renderer.upload(synth.cornell())
+    try:
+        jobs = [(sid, abi.render_opts(48, 40, 4, 8, seed=3)), (sid, abi.render_opts(32, 32, 2, 8, seed=4))]
+        first = renderer.render_batch(jobs, want_sq=True)
+        ids = [id(o) for pair in first for o in pair]
+        for pair in first:
+            for a in pair:
+                a.fill(-7.0)
+        again = renderer.render_batch(jobs, want_sq=True, outs=first)
+        assert [id(o) for pair in again for o in pair] == ids
+        fresh = renderer.render_batch(jobs, want_sq=True)
+        for (a, asq), (b, bsq) in zip(again, fresh):
+            assert np.array_equal(a, b) and np.array_equal(asq, bsq)
+        with pytest.raises(ValueError):
+            renderer.render_batch(jobs, outs=[first[1], first[0]])
+    finally:
+        renderer.free(sid)
